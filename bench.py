@@ -1,0 +1,153 @@
+"""Benchmark: scattering events/s of the fused forward kernel on one MI355X.
+
+Workload: 2-D elastic P-SV medium (vp = 6, vs = 3.46, f = 5, cv = 0.1, Gaussian
+ACF, lc = 0.5, Q = 500), cartesian frame, P point source, 1e7 particles,
+x, y = -50:0.5:50, 36 direction bins, t = 0:0.05:20 -- a 4.6 GB uint32
+histogram (x, y, direction, time, mode) resident in HBM.  One step = one full
+forward solve (histogram cleared, every particle born, propagated through all
+401 observation times and deposited), material tables already in HBM.
+
+Prints one JSON line (see the repository's benchmark contract) with a
+`roofline` object for the fused kernel (HBM bound: 8 algorithmic bytes per
+histogram deposit, a 4-byte read and write of the counter) and a
+`cpu_baseline` measured with the CPU oracle on a bounded sample of the same
+workload on this host's cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_DEPOSIT = 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--ppt", type=int, default=0)
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    p = os.path.join(ROOT, "profiles", "k_forward_config2_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(args, geometry, source, material, O):
+    import numpy as np
+    from oracle import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    # calibrate on a small sample, then size the measured sample to ~cpu_seconds
+    n_cal = 2000 * threads
+    t0 = time.perf_counter()
+    orc.forward(geometry, source, material, O, n_cal, seed=99, packet=2000, threads=threads)
+    t_cal = time.perf_counter() - t0
+    n = int(max(n_cal, min(args.particles, n_cal * args.cpu_seconds / max(t_cal, 1e-3))))
+    n = (n // (2000 * threads)) * 2000 * threads or n_cal
+    t0 = time.perf_counter()
+    _E, dep, nscat = orc.forward(geometry, source, material, O, n, seed=98, packet=2000, threads=threads,
+                                 return_counters=True)
+    t = time.perf_counter() - t0
+    return {"value": nscat / t, "unit": "scattering events/s", "cores": threads, "kind": "port",
+            "sample": f"{n} particles of the same workload (full 20 s, all 401 times), sub-step loop of "
+                      f"propagateParticleSmallDt restated in C (FP64, OpenMP over packets of 2000), {t:.1f} s; "
+                      f"{dep / t:.3e} deposits/s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
+                                   "and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from radiativetransfermontecarlo_amd import _abi, solver
+    from radiativetransfermontecarlo_amd.material import MaterialClass
+    from radiativetransfermontecarlo_amd.observation import initializeObservation
+    from tests import cases
+
+    dev = torch.device("cuda", 0)
+    geometry, source, material, observation = cases.config2(n=args.particles)
+    N = args.particles
+    material = MaterialClass.prepareSigma(material, 2)
+    O = initializeObservation(geometry, False, observation, N, allow_nd=True)
+    dmat = solver.DeviceMaterial(material, 2, dev)
+    E = torch.zeros(O.size, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(seed):
+        E.zero_()
+        cnt.zero_()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        keep = solver.forward_histogram(geometry, source, material, O, N, seed=seed, device=dev, E=E, counters=cnt,
+                                        dmat=dmat, block_size=args.block, particles_per_thread=args.ppt)
+        ev1.record(stream)
+        return ev0, ev1, keep
+
+    for w in range(args.warmup):
+        step(1000 + w)
+    torch.cuda.synchronize()
+    kern_ms = []
+    events = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev0, ev1, keep = step(s)
+        events.append((ev0, ev1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for ev0, ev1 in events:
+        kern_ms.append(ev0.elapsed_time(ev1))
+    c = cnt.cpu().numpy()
+    nscat = int(c[_abi.CNT_SCATTER])
+    ndep = int(c[_abi.CNT_DEPOSIT])
+    ms = 1000.0 * wall / args.steps
+    kavg = float(np.mean(kern_ms))
+    achieved = BYTES_PER_DEPOSIT * ndep / (kavg * 1e-3) / 1e9
+    out = {
+        "metric": "scattering events/s on 1 MI355X",
+        "value": nscat / (ms * 1e-3),
+        "unit": "scattering events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox4x32-10 particles; material tables from the reference's formulas)",
+        "config": {"workload": "2-D elastic P-SV, cartesian, 1e7 particles, Q=500, x,y=-50:0.5:50, 36 direction "
+                               "bins, t=0:0.05:20 (config 2)", "particles": N, "histogram_bytes": int(O.size) * 4,
+                   "deposits_per_step": ndep, "scatter_events_per_step": nscat,
+                   "deposits_per_s": ndep / (ms * 1e-3)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "k_forward", "kernel_ms": kavg},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, geometry, source, material, O)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,287 @@
+/*
+ * rtmc.h -- C ABI of librtmc, the MI355X (gfx950) particle-tracing core of
+ * radiativeTransfer(geometry, source, material, observation).
+ *
+ * Every entry point replaces one function (or loop) of the reference MATLAB
+ * code, cited below as file:line of cottereau/RadiativeTransferMonteCarlo.
+ * The reference has no FFI of its own: these are the symbols a MEX gateway
+ * (mexFunction) or a Python ctypes binding calls.
+ *
+ * Conventions
+ *  - Plain C types only. Bulk arrays are DEVICE pointers (hipMalloc'ed or
+ *    taken from a device tensor); the small descriptor structs live in host
+ *    memory and are passed by pointer.
+ *  - Arrays of N x 3 values are stored column-major exactly as MATLAB stores
+ *    an N x 3 double matrix: x[0..N-1] = x(:,1), x[N..2N-1] = x(:,2), ...
+ *  - Histograms are uint32 counts, column-major over their bin axes, then
+ *    time, then mode (mode 1 = S waves), as E in initializeObservation.m:133.
+ *  - Every function returns RTMC_OK (0) or a negative status; the message of
+ *    the last failure on the calling thread is returned by rtmc_last_error().
+ *    No C++ exception crosses the ABI.
+ *  - `stream` is a hipStream_t (NULL = the default stream). Calls are
+ *    stream-ordered; functions that must read a device value back (the
+ *    mean particle time, an overflow flag) synchronise that stream.
+ */
+#ifndef RTMC_H
+#define RTMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTMC_ABI_VERSION 1
+
+/* status codes */
+#define RTMC_OK 0
+#define RTMC_ERR_ARG (-1)         /* invalid argument / unsupported combination */
+#define RTMC_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define RTMC_ERR_DRAWS (-3)       /* replay stream shorter than the work needs */
+#define RTMC_ERR_NOMEM (-4)       /* device allocation failed */
+#define RTMC_ERR_TABLE (-5)       /* malformed interpolation table */
+
+#define RTMC_MAX_BND 8
+
+/* geometry.frame (initializeObservation.m:30-41) */
+#define RTMC_FRAME_CARTESIAN 0
+#define RTMC_FRAME_CYLINDRICAL 1
+#define RTMC_FRAME_SPHERICAL 2
+
+/* geometry.bnd(i).type (propagateParticleSmallDt.m:152-165) */
+#define RTMC_BND_REFLECTIVE 0
+#define RTMC_BND_ABSORBING 1
+
+/* source.type / source.direction / source.aperture (initializeParticle.m) */
+#define RTMC_SRC_POINT 0
+#define RTMC_SRC_PLANE 1
+#define RTMC_DIR_UNIFORM 0
+#define RTMC_DIR_OUTGOING 1
+#define RTMC_DIR_UPPER 2
+#define RTMC_APERTURE_RECTANGLE 0
+#define RTMC_APERTURE_CIRCLE 1
+
+/* interpolation-table extrapolation rule */
+#define RTMC_EXTRAP_LINEAR 0      /* griddedInterpolant default (MaterialClass.m:2462) */
+#define RTMC_EXTRAP_NEAREST 1     /* griddedInterpolant(...,'linear','nearest') (initializeParticle.m:201) */
+
+/* random-number source of a call */
+#define RTMC_DRAWS_REPLAY 0       /* recorded variates, one slot per particle per step */
+#define RTMC_DRAWS_PHILOX 1       /* Philox4x32-10 counter-based generator */
+
+/* arithmetic of the fused propagate kernel */
+#define RTMC_PREC_FP64 0
+#define RTMC_PREC_FP32_FLIGHT 1   /* FP32 direction/flight arithmetic, FP64 time */
+
+/* One boundary: geometry.bnd(i) = struct('dir',dir,'val',val,'type',type).
+ * dir 1..3 = plane normal to that axis at coordinate val; dir 4 = cylinder
+ * of radius val around the z axis (propagateParticleSmallDt.m:167-581). */
+typedef struct {
+    int32_t dir;
+    int32_t type;               /* RTMC_BND_* */
+    double val;
+} rtmc_boundary;
+
+typedef struct {
+    int32_t dimension;          /* 2 or 3 */
+    int32_t frame;              /* RTMC_FRAME_* */
+    int32_t n_bnd;              /* 0 .. RTMC_MAX_BND */
+    int32_t pad_;
+    rtmc_boundary bnd[RTMC_MAX_BND];
+} rtmc_geometry;
+
+/* A piecewise-linear table y(x) on strictly increasing knots x: the
+ * griddedInterpolant(x, y) of MaterialClass.prepareSigmaOne
+ * (MaterialClass.m:2459-2462) or of initializeParticle.m:201. */
+typedef struct {
+    int64_t n;                  /* number of knots (>= 2), 0 = table absent */
+    const double* x;            /* device, length n */
+    const double* y;            /* device, length n */
+    int32_t extrap;             /* RTMC_EXTRAP_* */
+    int32_t pad_;
+} rtmc_table;
+
+/* Prepared material: the fields of MaterialClass that the particle kernels
+ * read after MaterialClass.prepareSigma (MaterialClass.m:2379-2442). */
+typedef struct {
+    int32_t acoustics;          /* 1 acoustic, 0 elastic */
+    int32_t pad_;
+    double vp;                  /* acoustic: mat.v (prepareSigma sets vp = v) */
+    double vs;
+    double mft[2];              /* meanFreeTime of P and S (acoustic: both 1/Sigma); +Inf = no scattering */
+    double P2P;                 /* Sigma(1,1)/sum(Sigma(1,:)) */
+    double S2S;                 /* Sigma(2,2)/sum(Sigma(2,:)) */
+    double QP;                  /* +Inf = no intrinsic attenuation */
+    double QS;
+    double frequency;           /* mat.Frequency(1) */
+    rtmc_table invcdf[4];       /* [0] P->P (acoustic: the only table), [1] P->S, [2] S->P, [3] S->S */
+    /* Solid/air energy coefficients on the Zoeppritz angle grid
+     * (MaterialClass.m:1786-1830, ZoeppritzFluid.m), used by elastic
+     * reflective boundaries only. */
+    int64_t n_zoe;              /* 0 or number of angles (181) */
+    const double* zoe_deg;      /* device: incidence angles in degrees */
+    const double* zoe_Rpp;      /* device: E_Rpp */
+    const double* zoe_Rsvsv;    /* device: E_Rsvsv */
+} rtmc_material;
+
+/* Source description (initializeParticle.m:3-160, README.md:180-198). */
+typedef struct {
+    int32_t type;               /* RTMC_SRC_* */
+    int32_t direction;          /* point: RTMC_DIR_* */
+    int32_t aperture;           /* plane: RTMC_APERTURE_* */
+    int32_t polarization;       /* 0 = P (default), 1 = S */
+    double position[3];
+    double lambda;
+    double normal[3];           /* plane: unit propagation direction n */
+    double basis_u[3];          /* plane: first column of null(n) */
+    double basis_v[3];          /* plane: second column of null(n) */
+    double extent[2];           /* plane: rectangle sides [L1 L2] or circle radius in [0] */
+    rtmc_table radial;          /* point: user radial inverse CDF (n = 0: Gaussian radius) */
+} rtmc_source;
+
+/* Particle packet, structure of arrays (initializeParticle.m:163-170). */
+typedef struct {
+    int64_t n;
+    int32_t dimension;
+    int32_t pad_;
+    double* x;                  /* device, 3n, column-major */
+    double* dir;                /* device, 3n */
+    double* perp;               /* device, 3n */
+    uint8_t* p;                 /* device, n: 1 = P wave */
+    uint8_t* alive;             /* device, n */
+    double* t;                  /* device, n */
+} rtmc_particles;
+
+/* Random variates of one call.
+ * REPLAY: replay[(step * n_slots + slot) * n + i] is the variate of particle
+ * i at step `step` for draw site `slot` (the slot tables are listed with each
+ * entry point). n_steps is the number of steps the buffer holds.
+ * PHILOX: variate = uniform from philox4x32_10(counter, key) with
+ * counter = {i & 0xffffffff, (i >> 32) | slot << 16, step, stream},
+ * key = {seed & 0xffffffff, seed >> 32}, uniform = (bits53 + 1) * 2^-53. */
+typedef struct {
+    int32_t kind;               /* RTMC_DRAWS_* */
+    int32_t n_slots;            /* REPLAY: slots per step (must equal the entry point's count) */
+    const double* replay;       /* REPLAY: device buffer */
+    int64_t n_steps;            /* REPLAY: steps held */
+    uint64_t seed;              /* PHILOX */
+    uint64_t stream;            /* PHILOX: call identifier (low 32 bits used) */
+} rtmc_draws;
+
+/* Observation binning.  Variable v in {0: x / radius, 1: y / azimuth,
+ * 2: z / sin(elevation), 3: direction (dir_z in 3-D, acos(dir_z) in 2-D)}
+ * as computed by observeTime.m:15-45.  A particle is counted when every
+ * variable falls inside its edges (histcounts rule: e_k <= v < e_k+1, the
+ * last bin closed); bin k of variable v adds k * stride[v] to the index.
+ * For the reference's two-variable histograms (initializeObservation.m:79-130)
+ * the two binned variables have strides 1 and N1, the others one bin each.
+ * In 2-D variable 2 does not exist and is ignored. */
+typedef struct {
+    int32_t n_edges[4];         /* >= 2 each */
+    const double* edges[4];     /* device, non-decreasing */
+    int64_t stride[4];
+    int64_t time_stride;        /* index step of one observation time */
+    int64_t mode_stride;        /* index step of mode 1 (S); unused for acoustics */
+} rtmc_bins;
+
+/* Whole forward problem for the fused kernel (radiativeTransfer.m:20-158). */
+typedef struct {
+    rtmc_geometry geometry;
+    rtmc_source source;
+    rtmc_material material;
+    rtmc_bins bins;
+    int64_t n_particles;        /* particles of this call */
+    uint64_t particle_offset;   /* global id of the first particle (chunking / sharding) */
+    uint64_t seed;
+    int32_t n_times;            /* Nt, including t = 0 which receives no deposit */
+    int32_t time_steps;         /* material.timeSteps: 0 sub-stepped (default), 1 event-driven */
+    double dt;                  /* mean(diff(t)) (radiativeTransfer.m:37) */
+    int32_t precision;          /* RTMC_PREC_* */
+    int32_t block_size;         /* 0 = library default */
+    int32_t particles_per_thread; /* 0 = library default */
+    int32_t hist_mode;          /* 0 = auto, 1 = global atomics, 2 = LDS-privatised */
+} rtmc_problem;
+
+/* Information returned by rtmc_propagate_smalldt about its sub-steps. */
+typedef struct {
+    double T_eff;               /* T - mean(P.t) (propagateParticleSmallDt.m:4) */
+    double dt;                  /* sub-step (propagateParticleSmallDt.m:5-7) */
+    int64_t n_sub;              /* number of sub-steps */
+    double pkill_p;             /* 1 - exp(-omega dt / QP) */
+    double pkill_s;
+} rtmc_substep_info;
+
+/* Counters accumulated by rtmc_forward (device uint64[RTMC_NCOUNTERS]). */
+#define RTMC_CNT_SCATTER 0      /* scattering events */
+#define RTMC_CNT_DEPOSIT 1      /* histogram increments */
+#define RTMC_CNT_BOUNDARY 2     /* boundary interactions */
+#define RTMC_CNT_KILLED 3       /* particles removed by Q or absorbing boundaries */
+#define RTMC_NCOUNTERS 4
+
+/* ------------------------------------------------------------------ */
+
+/* Message of the last failure on this thread ("" if none). */
+const char* rtmc_last_error(void);
+
+/* RTMC_ABI_VERSION of the loaded library. */
+int rtmc_abi_version(void);
+
+/* Replaces initializeParticle(N, d, acoustics, source)
+ * (initializeParticle.m:1-171, local inverseCDF :175-203).
+ * Fills every array of P (P->n particles, P->dimension = d).
+ * Draw slots (one step): 0 randn radius (plane: randn depth), 1 u phi0,
+ * 2 u phi, 3 u theta0, 4 u theta, 5 u theta ('upper'), 6 u phi ('upper'),
+ * 7 u alpha; plane source: 1 u transverse 1 (circle: radius), 2 u transverse 2
+ * (circle: angle), 4 u polarisation angle.  n_slots = 8 (REPLAY).  Slot 0
+ * under PHILOX is a standard normal made by Box-Muller from the slot-0 and
+ * slot-8 uniforms. */
+int rtmc_init_particles(const rtmc_source* source, int32_t acoustics,
+                        const rtmc_draws* draws, rtmc_particles* P, void* stream);
+
+/* Replaces propagateParticleSmallDt(mat, geometry, P, T)
+ * (propagateParticleSmallDt.m:1-647, with MaterialClass.MaterialInterface
+ * MaterialClass.m:1572-1756 for elastic plane boundaries).
+ * T_eff: NaN = compute T - mean(P.t) on the device (as :4), otherwise used
+ * as given (replay of a recorded MATLAB value).  `info` may be NULL.
+ * Draw slots per sub-step: 0 u kill, 1 scatter (REPLAY: the poissrnd count;
+ * PHILOX: uniform compared with 1-exp(-dt/mft)), 2 azimuth (3-D uniform;
+ * 2-D: the randi([0 1]) bit), 3 u theta, 4 u P2P, 5 u S2S, then for
+ * boundary b: 6+2b first boundary draw (P: reflect-or-convert; S: SV/SH
+ * split), 7+2b second (SV: reflect-or-convert).  n_slots = 6 + 2 n_bnd. */
+int rtmc_propagate_smalldt(const rtmc_material* mat, const rtmc_geometry* geometry,
+                           rtmc_particles* P, double T, double T_eff,
+                           const rtmc_draws* draws, rtmc_substep_info* info,
+                           void* stream);
+
+/* Replaces propagateParticle(mat, P, T) (propagateParticle.m:1-96).
+ * Each particle loops until it reaches T; draw step = its loop iteration.
+ * Draw slots: 0 event count (REPLAY: the poissrnd count; PHILOX: Poisson
+ * sample by inversion of the slot-0 uniform), 1 minimum of the count's
+ * uniforms (timeNextJump :86-96; PHILOX draws it as 1-u^(1/count)),
+ * 2 azimuth, 3 u theta, 4 u P2P, 5 u S2S.  n_slots = 6 (REPLAY). */
+int rtmc_propagate_events(const rtmc_material* mat, rtmc_particles* P, double T,
+                          const rtmc_draws* draws, void* stream);
+
+/* Replaces E(:,:,it,:) = E(:,:,it,:) + observeTime(...)
+ * (observeTime.m:1-146 and radiativeTransfer.m:75-76): adds the histogram of
+ * the alive particles of P into E at time index `time_index`. */
+int rtmc_observe(const rtmc_geometry* geometry, const rtmc_bins* bins,
+                 int32_t acoustics, const rtmc_particles* P,
+                 uint32_t* E, int64_t time_index, void* stream);
+
+/* Replaces the packet x time loop of radiativeTransfer
+ * (radiativeTransfer.m:40-158): births, propagates and observes
+ * prob->n_particles particles in one persistent kernel with Philox draws,
+ * adding their counts into E (device, caller-zeroed, see rtmc_bins).
+ * time_steps 0 samples exactly the sub-step process of
+ * propagateParticleSmallDt (scattering and Q removal tested once per
+ * sub-step), event to event; time_steps 1 is the event-driven process of
+ * propagateParticle.  counters: NULL or device uint64[RTMC_NCOUNTERS]. */
+int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* counters,
+                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTMC_H */

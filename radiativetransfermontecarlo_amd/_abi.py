@@ -1,0 +1,250 @@
+"""ctypes mirror of include/rtmc.h and the loader of librtmc.so.
+
+The library is built in-tree (``__graft_entry__.build()``) next to this file.
+Loading fails loudly when it is missing: there is no CPU fallback for the
+particle kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+ABI_VERSION = 1
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_DRAWS = -3
+ERR_NOMEM = -4
+ERR_TABLE = -5
+
+MAX_BND = 8
+
+FRAME_CARTESIAN = 0
+FRAME_CYLINDRICAL = 1
+FRAME_SPHERICAL = 2
+FRAMES = {"cartesian": FRAME_CARTESIAN, "cylindrical": FRAME_CYLINDRICAL, "spherical": FRAME_SPHERICAL}
+
+BND_REFLECTIVE = 0
+BND_ABSORBING = 1
+
+SRC_POINT = 0
+SRC_PLANE = 1
+DIR_UNIFORM = 0
+DIR_OUTGOING = 1
+DIR_UPPER = 2
+APERTURE_RECTANGLE = 0
+APERTURE_CIRCLE = 1
+
+EXTRAP_LINEAR = 0
+EXTRAP_NEAREST = 1
+
+DRAWS_REPLAY = 0
+DRAWS_PHILOX = 1
+
+PREC_FP64 = 0
+PREC_FP32_FLIGHT = 1
+
+CNT_SCATTER = 0
+CNT_DEPOSIT = 1
+CNT_BOUNDARY = 2
+CNT_KILLED = 3
+NCOUNTERS = 4
+
+# draw-slot counts of the entry points (include/rtmc.h)
+INIT_SLOTS = 8
+EVENT_SLOTS = 6
+
+
+def smalldt_slots(n_bnd: int) -> int:
+    return 6 + 2 * n_bnd
+
+
+class Boundary(C.Structure):
+    _fields_ = [("dir", C.c_int32), ("type", C.c_int32), ("val", C.c_double)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("dimension", C.c_int32),
+        ("frame", C.c_int32),
+        ("n_bnd", C.c_int32),
+        ("pad_", C.c_int32),
+        ("bnd", Boundary * MAX_BND),
+    ]
+
+
+class Table(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("x", C.c_void_p),
+        ("y", C.c_void_p),
+        ("extrap", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("acoustics", C.c_int32),
+        ("pad_", C.c_int32),
+        ("vp", C.c_double),
+        ("vs", C.c_double),
+        ("mft", C.c_double * 2),
+        ("P2P", C.c_double),
+        ("S2S", C.c_double),
+        ("QP", C.c_double),
+        ("QS", C.c_double),
+        ("frequency", C.c_double),
+        ("invcdf", Table * 4),
+        ("n_zoe", C.c_int64),
+        ("zoe_deg", C.c_void_p),
+        ("zoe_Rpp", C.c_void_p),
+        ("zoe_Rsvsv", C.c_void_p),
+    ]
+
+
+class Source(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("direction", C.c_int32),
+        ("aperture", C.c_int32),
+        ("polarization", C.c_int32),
+        ("position", C.c_double * 3),
+        ("lambda_", C.c_double),
+        ("normal", C.c_double * 3),
+        ("basis_u", C.c_double * 3),
+        ("basis_v", C.c_double * 3),
+        ("extent", C.c_double * 2),
+        ("radial", Table),
+    ]
+
+
+class Particles(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("dimension", C.c_int32),
+        ("pad_", C.c_int32),
+        ("x", C.c_void_p),
+        ("dir", C.c_void_p),
+        ("perp", C.c_void_p),
+        ("p", C.c_void_p),
+        ("alive", C.c_void_p),
+        ("t", C.c_void_p),
+    ]
+
+
+class Draws(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("n_slots", C.c_int32),
+        ("replay", C.c_void_p),
+        ("n_steps", C.c_int64),
+        ("seed", C.c_uint64),
+        ("stream", C.c_uint64),
+    ]
+
+
+class Bins(C.Structure):
+    _fields_ = [
+        ("n_edges", C.c_int32 * 4),
+        ("edges", C.c_void_p * 4),
+        ("stride", C.c_int64 * 4),
+        ("time_stride", C.c_int64),
+        ("mode_stride", C.c_int64),
+    ]
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("geometry", Geometry),
+        ("source", Source),
+        ("material", Material),
+        ("bins", Bins),
+        ("n_particles", C.c_int64),
+        ("particle_offset", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("n_times", C.c_int32),
+        ("time_steps", C.c_int32),
+        ("dt", C.c_double),
+        ("precision", C.c_int32),
+        ("block_size", C.c_int32),
+        ("particles_per_thread", C.c_int32),
+        ("hist_mode", C.c_int32),
+    ]
+
+
+class SubstepInfo(C.Structure):
+    _fields_ = [
+        ("T_eff", C.c_double),
+        ("dt", C.c_double),
+        ("n_sub", C.c_int64),
+        ("pkill_p", C.c_double),
+        ("pkill_s", C.c_double),
+    ]
+
+
+LIB_NAME = "librtmc.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+# exported symbols of include/rtmc.h (checked by the CPU test suite)
+EXPORTS = (
+    "rtmc_last_error",
+    "rtmc_abi_version",
+    "rtmc_init_particles",
+    "rtmc_propagate_smalldt",
+    "rtmc_propagate_events",
+    "rtmc_observe",
+    "rtmc_forward",
+)
+
+_lib = None
+
+
+class RTMCError(RuntimeError):
+    """A librtmc call returned a non-zero status."""
+
+
+def _declare(lib):
+    P = C.POINTER
+    lib.rtmc_last_error.restype = C.c_char_p
+    lib.rtmc_last_error.argtypes = []
+    lib.rtmc_abi_version.restype = C.c_int
+    lib.rtmc_abi_version.argtypes = []
+    lib.rtmc_init_particles.restype = C.c_int
+    lib.rtmc_init_particles.argtypes = [P(Source), C.c_int32, P(Draws), P(Particles), C.c_void_p]
+    lib.rtmc_propagate_smalldt.restype = C.c_int
+    lib.rtmc_propagate_smalldt.argtypes = [P(Material), P(Geometry), P(Particles), C.c_double,
+                                           C.c_double, P(Draws), P(SubstepInfo), C.c_void_p]
+    lib.rtmc_propagate_events.restype = C.c_int
+    lib.rtmc_propagate_events.argtypes = [P(Material), P(Particles), C.c_double, P(Draws), C.c_void_p]
+    lib.rtmc_observe.restype = C.c_int
+    lib.rtmc_observe.argtypes = [P(Geometry), P(Bins), C.c_int32, P(Particles), C.c_void_p,
+                                 C.c_int64, C.c_void_p]
+    lib.rtmc_forward.restype = C.c_int
+    lib.rtmc_forward.argtypes = [P(Problem), C.c_void_p, C.c_void_p, C.c_void_p]
+    return lib
+
+
+def load(path: str | None = None):
+    """Load librtmc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RTMCError(
+            f"{p} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = _declare(C.CDLL(p))
+    v = lib.rtmc_abi_version()
+    if v != ABI_VERSION:
+        raise RTMCError(f"librtmc ABI version {v}, expected {ABI_VERSION}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status: int, what: str) -> None:
+    if status != OK:
+        msg = load().rtmc_last_error().decode(errors="replace")
+        raise RTMCError(f"{what} failed with status {status}: {msg}")

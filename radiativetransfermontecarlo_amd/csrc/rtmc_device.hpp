@@ -1,0 +1,496 @@
+// rtmc_device.hpp -- device building blocks shared by the librtmc kernels
+// (gfx950).  Physics follows cottereau/RadiativeTransferMonteCarlo; each
+// routine cites the reference lines it implements.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rtmc.h"
+
+namespace rtmc {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kTwoPi = 6.283185307179586;
+constexpr double kEps = 2.220446049250313e-16;  // MATLAB eps
+
+// ------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Multipliers and Weyl constants
+// are those of the paper.
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+    uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+    U4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// 53 random bits -> (0, 1]
+__device__ __forceinline__ double u53(uint32_t w0, uint32_t w1) {
+  uint64_t b = ((((uint64_t)w1) << 32) | w0) >> 11;
+  return (double)(b + 1) * (1.0 / 9007199254740992.0);
+}
+
+// 24 random bits -> (0, 1] in FP32
+__device__ __forceinline__ float u24(uint32_t w) {
+  return (float)((w >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// Variates of the function-level entry points (rtmc_draws in rtmc.h).
+struct SlotDraws {
+  int kind;
+  int n_slots;
+  const double* replay;
+  int64_t n_steps;
+  uint32_t k0, k1, stream;
+  int64_t n;
+
+  __device__ __forceinline__ double operator()(int64_t i, int slot, int64_t step) const {
+    if (kind == RTMC_DRAWS_REPLAY) return replay[(step * n_slots + slot) * n + i];
+    U4 c;
+    c.x = (uint32_t)((uint64_t)i & 0xffffffffu);
+    c.y = (uint32_t)(((uint64_t)i >> 32) & 0xffffu) | ((uint32_t)slot << 16);
+    c.z = (uint32_t)step;
+    c.w = stream;
+    U4 r = philox10(c, k0, k1);
+    return u53(r.x, r.y);
+  }
+};
+
+// ------------------------------------------------------------------
+// Small vector helpers.  Sums are left to right, as MATLAB's sum(.,2).
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double sumsq3(const double a[3]) { return (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]; }
+__device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+__device__ __forceinline__ double sgn(double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); }
+__device__ __forceinline__ double acosd(double c) { return acos(c) * 180.0 / kPi; }
+
+// ------------------------------------------------------------------
+// griddedInterpolant(x, y)(q): linear inside, linear or nearest outside
+// (MaterialClass.m:2462, initializeParticle.m:201, interp1(...,'extrap')
+// MaterialClass.m:1674).  Interval = largest k <= n-2 with x[k] <= q.
+__device__ __forceinline__ double table_eval(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                                             int extrap, double q) {
+  if (q != q) return q;
+  int64_t k;
+  if (q < x[0]) {
+    if (extrap == RTMC_EXTRAP_NEAREST) return y[0];
+    k = 0;
+  } else if (q > x[n - 1]) {
+    if (extrap == RTMC_EXTRAP_NEAREST) return y[n - 1];
+    k = n - 2;
+  } else {
+    int64_t lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+      int64_t mid = lo + ((hi - lo) >> 1);
+      if (x[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    k = lo;
+  }
+  double t = (q - x[k]) / (x[k + 1] - x[k]);
+  return y[k] + t * (y[k + 1] - y[k]);
+}
+
+// Same lookup, restricted to the knot range [lo, hi] known to bracket q
+// (guide-table search: lo = largest knot <= q is inside [lo, hi]).
+__device__ __forceinline__ double table_eval_bracket(const double* __restrict__ x, const double* __restrict__ y,
+                                                     int64_t n, int extrap, double q, int64_t lo, int64_t hi) {
+  if (q != q) return q;
+  int64_t k;
+  if (q < x[0]) {
+    if (extrap == RTMC_EXTRAP_NEAREST) return y[0];
+    k = 0;
+  } else if (q > x[n - 1]) {
+    if (extrap == RTMC_EXTRAP_NEAREST) return y[n - 1];
+    k = n - 2;
+  } else {
+    if (hi > n - 1) hi = n - 1;
+    if (lo < 0) lo = 0;
+    if (lo > n - 2) lo = n - 2;
+    // widen if the bracket does not hold (defensive; never needed for a valid guide)
+    if (x[lo] > q) lo = 0;
+    if (hi <= lo || !(x[hi] > q)) hi = n - 1;
+    while (hi - lo > 1) {
+      int64_t mid = lo + ((hi - lo) >> 1);
+      if (x[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    k = lo;
+  }
+  double t = (q - x[k]) / (x[k + 1] - x[k]);
+  return y[k] + t * (y[k + 1] - y[k]);
+}
+
+struct TableDev {
+  const double* x;
+  const double* y;
+  int64_t n;
+  int extrap;
+};
+
+// ------------------------------------------------------------------
+// Material as the kernels read it.
+struct MatDev {
+  int acoustics;
+  double vp, vs;
+  double mftP, mftS;
+  double P2P, S2S;
+  TableDev inv[4];
+  const double* zdeg;
+  const double* zRpp;
+  const double* zRss;
+  int64_t nz;
+};
+
+// ------------------------------------------------------------------
+// Boundary interaction of one particle, given that it crossed boundary B
+// during the current sub-step (propagateParticleSmallDt.m:167-581).
+// Returns true if the particle was absorbed.  uA, uB: the two boundary
+// variates (P: reflect-or-convert; S: SV/SH split; SV: reflect-or-convert).
+__device__ __forceinline__ bool plane_crossed(const rtmc_boundary& B, const double x[3], const double x1[3]) {
+  int b = B.dir - 1;
+  return (x[b] - B.val) * (x1[b] - B.val) < 0;  // :171-172
+}
+
+__device__ __forceinline__ bool cyl_outside(const rtmc_boundary& B, const double x[3]) {
+  double r = sqrt(x[0] * x[0] + x[1] * x[1]);  // :328-329
+  return (r - B.val) > 0;                      // :332-334
+}
+
+// MaterialClass.MaterialInterface branch at plane of normal e_b
+// (MaterialClass.m:1587-1735): 0 P->P, 1 P->SV, 2 SH, 3 SV->SV, 4 SV->P.
+__device__ __forceinline__ int interface_plane(const MatDev& M, int b, const double d[3], const double q[3],
+                                               int isP, double uA, double uB) {
+  double nv[3] = {0.0, 0.0, 0.0};
+  nv[b] = 1.0;
+  if (isP) {
+    double a = fabs(dot3(d, nv));  // :1597-1599
+    if (a > 1) a = 1;
+    double inc = acosd(a);                                               // :1600
+    double R = table_eval(M.zdeg, M.zRpp, M.nz, RTMC_EXTRAP_LINEAR, inc);  // :1679
+    return (uA <= R) ? 0 : 1;                                            // :1681-1682
+  }
+  double eSH[3];
+  cross3(d, nv, eSH);  // :1613
+  double nsh = sqrt(sumsq3(eSH));
+  if (nsh < kEps) {  // :1616-1625
+    double t[3] = {1.0, 0.0, 0.0};
+    double tn = dot3(t, nv);
+    for (int k = 0; k < 3; ++k) t[k] = t[k] - tn * nv[k];
+    for (int k = 0; k < 3; ++k) eSH[k] = t[k];
+    nsh = sqrt(sumsq3(t));
+  }
+  for (int k = 0; k < 3; ++k) eSH[k] = eSH[k] / nsh;
+  double eSV[3];
+  cross3(eSH, d, eSV);  // :1630-1631
+  double nsv = sqrt(sumsq3(eSV));
+  for (int k = 0; k < 3; ++k) eSV[k] = eSV[k] / nsv;
+  double aSV = dot3(q, eSV), aSH = dot3(q, eSH);  // :1634-1643
+  double ESV = aSV * aSV, ESH = aSH * aSH;
+  double Etot = ESV + ESH + kEps;
+  double probSV = ESV / Etot;
+  if (!(uA < probSV)) return 2;  // :1646, SH always reflects :1720
+  double a = fabs(dot3(d, nv));
+  if (a > 1) a = 1;
+  double inc = acosd(a);
+  double R = table_eval(M.zdeg, M.zRss, M.nz, RTMC_EXTRAP_LINEAR, inc);  // :1700
+  return (uB <= R) ? 3 : 4;                                            // :1702-1703
+}
+
+// Applies the reflection of a particle that crossed plane boundary B
+// (acoustic :184-187; elastic :190-321).
+__device__ __forceinline__ void plane_reflect(const MatDev& M, const rtmc_boundary& B, double x[3], double d[3],
+                                              double q[3], int& isP, double uA, double uB) {
+  int b = B.dir - 1;
+  if (M.acoustics) {
+    x[b] = (2 * B.val) - x[b];
+    d[b] = -d[b];
+    return;
+  }
+  int br = interface_plane(M, b, d, q, isP, uA, uB);
+  x[b] = (2 * B.val) - x[b];  // :198
+  if (br == 0 || br == 2 || br == 3) {
+    d[b] = -d[b];
+    q[b] = -q[b];
+  } else if (br == 1) {  // P -> SV :212-248
+    isP = 0;
+    double ratio = M.vs / M.vp;
+    double dn = d[b];
+    double t[3] = {d[0], d[1], d[2]};
+    t[b] = 0.0;
+    for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
+    double sin2 = sumsq3(t);
+    double cv = sqrt(fmax(0.0, 1.0 - sin2));
+    t[b] = -sgn(dn) * cv;
+    double nn = sqrt(sumsq3(t));
+    for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
+    double nv[3] = {0.0, 0.0, 0.0};
+    nv[b] = 1.0;
+    double ax[3];
+    cross3(d, nv, ax);
+    double na = sqrt(sumsq3(ax));
+    for (int k = 0; k < 3; ++k) ax[k] = ax[k] / na;
+    cross3(ax, d, q);
+  } else {  // SV -> P :274-320
+    isP = 1;
+    double ratio = M.vp / M.vs;
+    double dn = d[b];
+    double t[3] = {d[0], d[1], d[2]};
+    t[b] = 0.0;
+    for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
+    double sin2 = sumsq3(t);
+    if (sin2 <= 1) {
+      double cv = sqrt(1 - sin2);
+      t[b] = -sgn(dn) * cv;
+      double nn = sqrt(sumsq3(t));
+      for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
+    } else {
+      isP = 0;  // total reflection fallback :313-319
+      d[b] = -d[b];
+    }
+  }
+}
+
+__device__ __forceinline__ void reflect_dq(double d[3], double q[3], const double nv[3]) {
+  double dtn = dot3(d, nv);
+  double ptn = dot3(q, nv);
+  for (int k = 0; k < 3; ++k) {
+    d[k] = d[k] - 2 * dtn * nv[k];
+    q[k] = q[k] - 2 * ptn * nv[k];
+  }
+}
+
+// Applies the reflection of a particle outside cylinder B
+// (propagateParticleSmallDt.m:344-578).
+__device__ __forceinline__ void cyl_reflect(const MatDev& M, const rtmc_boundary& B, double x[3], double d[3],
+                                            double q[3], int& isP, double uA, double uB) {
+  double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+  double ov = r - B.val;
+  double nv[3] = {x[0] / r, x[1] / r, 0.0};
+  for (int k = 0; k < 3; ++k) x[k] = x[k] - 2 * ov * nv[k];  // :352
+  if (M.acoustics) {                                         // :354-359
+    double dp = dot3(d, nv);
+    for (int k = 0; k < 3; ++k) d[k] = d[k] - 2 * dp * nv[k];
+    return;
+  }
+  if (isP) {  // :373-455
+    double ct = dot3(d, nv);
+    double thd = acosd(fmin(1.0, fmax(-1.0, ct)));
+    double R = table_eval(M.zdeg, M.zRpp, M.nz, RTMC_EXTRAP_NEAREST, thd);
+    if (uA <= R) {
+      reflect_dq(d, q, nv);
+    } else {
+      isP = 0;
+      double eta = M.vs / M.vp;
+      double ddn = dot3(d, nv);
+      double dt_[3], dnt[3];
+      for (int k = 0; k < 3; ++k) {
+        double dnv = ddn * nv[k];
+        dt_[k] = d[k] - dnv;
+      }
+      for (int k = 0; k < 3; ++k) dnt[k] = eta * dt_[k];
+      double sin2 = sumsq3(dnt);
+      double cv = sqrt(fmax(0.0, 1 - sin2));
+      double dnew[3];
+      for (int k = 0; k < 3; ++k) dnew[k] = dnt[k] - cv * nv[k];
+      double nn = sqrt(sumsq3(dnew));
+      for (int k = 0; k < 3; ++k) dnew[k] = dnew[k] / nn;
+      for (int k = 0; k < 3; ++k) d[k] = dnew[k];
+      double ax[3];
+      cross3(dnew, nv, ax);
+      double al = sqrt(sumsq3(ax));
+      if (al < 1e-6) {
+        double e1[3] = {1.0, 0.0, 0.0};
+        cross3(dnew, e1, ax);
+      }
+      double an = sqrt(sumsq3(ax));
+      for (int k = 0; k < 3; ++k) ax[k] = ax[k] / an;
+      cross3(ax, dnew, q);
+    }
+  } else {  // :458-576
+    double ai[3];
+    cross3(d, nv, ai);
+    double an = sqrt(sumsq3(ai));
+    for (int k = 0; k < 3; ++k) ai[k] = ai[k] / an;
+    double shc = dot3(q, ai);
+    double prob_sh = shc * shc;
+    if (uA < prob_sh) {
+      reflect_dq(d, q, nv);
+    } else {
+      double ct = dot3(d, nv);
+      double thd = acosd(fmin(1.0, fmax(-1.0, ct)));
+      double R = table_eval(M.zdeg, M.zRss, M.nz, RTMC_EXTRAP_NEAREST, thd);
+      if (uB <= R) {
+        reflect_dq(d, q, nv);
+      } else {
+        isP = 1;
+        double eta = M.vp / M.vs;
+        double ddn = dot3(d, nv);
+        double dt_[3], dnt[3];
+        for (int k = 0; k < 3; ++k) {
+          double dnv = ddn * nv[k];
+          dt_[k] = d[k] - dnv;
+        }
+        for (int k = 0; k < 3; ++k) dnt[k] = eta * dt_[k];
+        double sin2 = sumsq3(dnt);
+        if (sin2 <= 1) {
+          double cv = sqrt(1 - sin2);
+          double dnew[3];
+          for (int k = 0; k < 3; ++k) dnew[k] = dnt[k] - cv * nv[k];
+          double nn = sqrt(sumsq3(dnew));
+          for (int k = 0; k < 3; ++k) d[k] = dnew[k] / nn;
+        } else {
+          isP = 0;
+          reflect_dq(d, q, nv);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// Scattering of one particle (propagateParticleSmallDt.m:604-639,
+// propagateParticle.m:33-72).  bit2d: the 2-D randi([0 1]) sign bit;
+// uphi: 3-D azimuth uniform; uth: polar-angle uniform; uconv: the P2P (P)
+// or S2S (S) uniform.  Returns theta.
+template <int DIM>
+__device__ __forceinline__ double scatter_angles(const MatDev& M, double d[3], double q[3], int& isP, bool bit2d,
+                                                 double uphi, double uth, double uconv) {
+  if (DIM == 2) {
+    double s = bit2d ? -1.0 : 1.0;
+    for (int k = 0; k < 3; ++k) q[k] = s * q[k];
+  } else {
+    double phi = kTwoPi * uphi;
+    double c[3];
+    cross3(d, q, c);
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    for (int k = 0; k < 3; ++k) q[k] = cp * q[k] + sp * c[k];
+  }
+  if (M.acoustics) {
+    return table_eval(M.inv[0].x, M.inv[0].y, M.inv[0].n, M.inv[0].extrap, uth);
+  }
+  int from_p = isP;
+  if (from_p) {
+    if (uconv > M.P2P) isP = 0;
+  } else {
+    if (uconv > M.S2S) isP = 1;
+  }
+  int tab = from_p ? (isP ? 0 : 1) : (isP ? 2 : 3);
+  const TableDev& T = M.inv[tab];
+  return table_eval(T.x, T.y, T.n, T.extrap, uth);
+}
+
+// dir <- cos(th) dir + sin(th) perp, perp <- -sin(th) dir + cos(th) perp,
+// both renormalised (propagateParticleSmallDt.m:635-639).
+__device__ __forceinline__ void rotate_renorm(double d[3], double q[3], double th) {
+  double st, ct;
+  sincos(th, &st, &ct);
+  double nd[3], np[3];
+  for (int k = 0; k < 3; ++k) nd[k] = ct * d[k] + st * q[k];
+  double nrm = sqrt(sumsq3(nd));
+  for (int k = 0; k < 3; ++k) nd[k] = nd[k] / nrm;
+  for (int k = 0; k < 3; ++k) np[k] = -st * d[k] + ct * q[k];
+  double nrp = sqrt(sumsq3(np));
+  for (int k = 0; k < 3; ++k) np[k] = np[k] / nrp;
+  for (int k = 0; k < 3; ++k) {
+    d[k] = nd[k];
+    q[k] = np[k];
+  }
+}
+
+__device__ __forceinline__ void renorm(double d[3], double q[3]) {
+  double nrm = sqrt(sumsq3(d));
+  for (int k = 0; k < 3; ++k) d[k] = d[k] / nrm;
+  double nrp = sqrt(sumsq3(q));
+  for (int k = 0; k < 3; ++k) q[k] = q[k] / nrp;
+}
+
+// ------------------------------------------------------------------
+// Observation variables (observeTime.m:15-45): 0 x|r, 1 y|azimuth,
+// 2 z|sin(elevation), 3 direction.  `need` masks the ones to compute.
+__device__ __forceinline__ void obs_vars(int dim, int frame, const double X[3], double dz, unsigned need,
+                                         double v[4]) {
+  double r = 0.0, th = 0.0, ph = 0.0;
+  bool polar = frame != RTMC_FRAME_CARTESIAN;
+  if (dim == 2) {
+    if (polar && (need & 1u)) r = hypot(X[0], X[1]);
+    if (polar && (need & 2u)) th = atan2(X[1], X[0]);
+  } else if (frame == RTMC_FRAME_CYLINDRICAL) {
+    if (need & 1u) r = hypot(X[0], X[1]);
+    if (need & 2u) th = atan2(X[1], X[0]);
+  } else if (frame == RTMC_FRAME_SPHERICAL) {
+    double hxy = 0.0;
+    if (need & 5u) hxy = hypot(X[0], X[1]);
+    if (need & 1u) r = hypot(hxy, X[2]);
+    if (need & 2u) th = atan2(X[1], X[0]);
+    if (need & 4u) ph = sin(atan2(X[2], hxy));
+  }
+  double cp = dz;
+  if (cp > 1) cp = 1;
+  if (cp < -1) cp = -1;
+  double psi = 0.0;
+  if (need & 8u) psi = (dim == 3) ? cp : acos(cp);
+  if (frame == RTMC_FRAME_CARTESIAN) {
+    v[0] = X[0];
+    v[1] = X[1];
+    v[2] = X[2];
+  } else if (frame == RTMC_FRAME_CYLINDRICAL) {
+    v[0] = r;
+    v[1] = th;
+    v[2] = X[2];
+  } else {
+    v[0] = r;
+    v[1] = th;
+    v[2] = ph;
+  }
+  v[3] = psi;
+}
+
+// histcounts bin: e_k <= v < e_k+1, last bin closed; -1 outside or NaN.
+__device__ __forceinline__ int64_t bin_search(const double* __restrict__ e, int ne, double v) {
+  if (!(v >= e[0] && v <= e[ne - 1])) return -1;
+  int64_t lo = 0, hi = ne - 1;
+  while (hi - lo > 1) {
+    int64_t mid = lo + ((hi - lo) >> 1);
+    if (e[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Same bin with a guess from the mean spacing, corrected against the true
+// edges (exact for any non-decreasing edges; O(1) steps when near-uniform).
+__device__ __forceinline__ int bin_guess(const double* __restrict__ e, int ne, double e0, double inv_w, double v) {
+  if (!(v >= e0 && v <= e[ne - 1])) return -1;
+  int nb = ne - 1;
+  double g = (v - e0) * inv_w;
+  int k = (g >= 0.0 && g < (double)nb) ? (int)g : (g >= (double)nb ? nb - 1 : 0);
+  while (k > 0 && v < e[k]) --k;
+  while (k < nb - 1 && v >= e[k + 1]) ++k;
+  return k;
+}
+
+}  // namespace rtmc

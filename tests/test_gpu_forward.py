@@ -1,0 +1,146 @@
+"""The fused rtmc_forward kernel against the CPU oracle (statistical parity).
+
+The fused kernel samples the sub-step process of propagateParticleSmallDt
+event to event with its own Philox streams; the oracle runs the reference's
+sub-step loop literally.  Different random streams: the histograms must agree
+bin by bin within Monte Carlo error (tests/stats.py).  Exact invariants:
+with Q = Inf and bins covering every reachable position, obs.energy(t > 0)
+is exactly 1.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from radiativetransfermontecarlo_amd import _abi, solver
+from radiativetransfermontecarlo_amd.material import MaterialClass
+from radiativetransfermontecarlo_amd.observation import initializeObservation
+from tests import cases
+from tests.stats import assert_same_distribution
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(geometry, source, material, observation, n_gpu, n_cpu, time_steps=0, seed=11, hist_mode=0,
+              allow_nd=False):
+    d = geometry['dimension']
+    material = MaterialClass.prepareSigma(material, d)
+    material.timeSteps = time_steps
+    O = initializeObservation(geometry, bool(material.acoustics), observation, n_gpu, allow_nd=allow_nd)
+    E, cnt, _k = solver.forward_histogram(geometry, source, material, O, n_gpu, seed=seed, device='cuda',
+                                          time_steps=time_steps, hist_mode=hist_mode)
+    Eg = E.cpu().numpy().view(np.uint32)
+    Eo, _ = orc.forward(geometry, source, material, O, n_cpu, seed=seed + 1, time_steps=time_steps,
+                        packet=10_000)
+    return O, Eg, Eo, cnt.cpu().numpy()
+
+
+@pytest.mark.parametrize("hist_mode", [0, 1])
+def test_config1_shape_statistics(gpu, hist_mode):
+    geometry, source, material, observation = cases.config1()
+    observation = dict(observation, time=np.arange(0, 2 + 1e-9, 0.02))
+    O, Eg, Eo, cnt = _run_both(geometry, source, material, observation, 1_000_000, 100_000, hist_mode=hist_mode)
+    # every particle is inside r < 6 up to t = 2: each time slice holds all of them
+    per_t = Eg.reshape(O.shape, order='F').sum(axis=(0, 1, 3))
+    assert np.all(per_t[1:] == 1_000_000) and per_t[0] == 0
+    assert cnt[_abi.CNT_DEPOSIT] == 1_000_000 * (O.Nt - 1)
+    assert_same_distribution(Eg, 1_000_000, Eo, 100_000, "config1")
+
+
+def test_elastic_2d_q_statistics(gpu):
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-3, 3, 25)
+    observation = {'x': e, 'y': e, 'time': np.arange(0, 0.8 + 1e-9, 0.05)}
+    m = cases.elastic_iso(2)
+    m.Q = [40.0, 20.0]
+    m.Frequency = 1.0
+    O, Eg, Eo, cnt = _run_both(geometry, source, m, observation, 1_000_000, 60_000)
+    assert cnt[_abi.CNT_KILLED] > 0
+    assert_same_distribution(Eg, 1_000_000, Eo, 60_000, "elastic 2-D with Q")
+
+
+def test_elastic_half_space_statistics(gpu):
+    """Free surface z = 0 (MaterialInterface + Zoeppritz), cylindrical frame."""
+    geometry = {'dimension': 3, 'frame': 'cylindrical', 'bnd': [{'dir': 3, 'val': 0.0}]}
+    source = {'lambda': 0.05, 'position': [0, 0, -0.3]}
+    observation = {'x': np.linspace(0, 4, 21), 'z': np.linspace(-4, 0, 9), 'time': np.arange(0, 0.6 + 1e-9, 0.05)}
+    m = cases.elastic_iso(3, rho=2000)
+    O, Eg, Eo, cnt = _run_both(geometry, source, m, observation, 1_000_000, 40_000)
+    assert cnt[_abi.CNT_BOUNDARY] > 0
+    # nothing ever sits above the free surface
+    assert_same_distribution(Eg, 1_000_000, Eo, 40_000, "elastic half-space")
+
+
+def test_acoustic_cylinder_and_absorbing_statistics(gpu):
+    geometry = {'dimension': 3, 'frame': 'cylindrical',
+                'bnd': [{'dir': 4, 'val': 1.0}, {'dir': 3, 'val': 1.5, 'type': 'absorbing'}]}
+    source = {'lambda': 0.05}
+    observation = {'x': np.linspace(0, 1.2, 13), 'z': np.linspace(-3, 3, 13), 'time': np.arange(0, 2 + 1e-9, 0.1)}
+    m = cases.iso_acoustic(3, v=2.0)
+    O, Eg, Eo, cnt = _run_both(geometry, source, m, observation, 1_000_000, 60_000)
+    assert cnt[_abi.CNT_KILLED] > 0
+    # the cylinder keeps every particle at r <= 1
+    Eh = Eg.reshape(O.shape, order='F')
+    assert Eh[10:].sum() == 0
+    assert_same_distribution(Eg, 1_000_000, Eo, 60_000, "cylinder + absorbing plane")
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_event_driven_statistics(gpu, d):
+    """material.timeSteps = 1: exponential free flight (propagateParticle.m)."""
+    geometry = {'dimension': d, 'frame': 'spherical'}
+    source = {'lambda': 0.02}
+    observation = {'x': np.linspace(0, 4, 41), 'time': np.arange(0, 1.5 + 1e-9, 0.05)}
+    m = MaterialClass({'dimension': d}, 10, True, 2, [0.1, 0.1], 0.0, 'exp', 0.1)
+    O, Eg, Eo, cnt = _run_both(geometry, source, m, observation, 1_000_000, 60_000, time_steps=1)
+    assert cnt[_abi.CNT_SCATTER] > 0
+    assert_same_distribution(Eg, 1_000_000, Eo, 60_000, f"event-driven {d}-D")
+
+
+def test_three_binned_variables(gpu):
+    """N-D histogram (x, y, directions) beyond the reference's two variables: its
+    2-D marginal equals the reference-compatible (x, y) histogram in law."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-2, 2, 17)
+    m = cases.elastic_iso(2)
+    obs3 = {'x': e, 'y': e, 'directions': np.linspace(0, np.pi, 5), 'time': np.arange(0, 0.5 + 1e-9, 0.05)}
+    O3, Eg3, _, _ = _run_both(geometry, source, m, obs3, 500_000, 1000, allow_nd=True)
+    obs2 = dict(obs3, directions=[0, np.pi])
+    O2, Eg2, Eo2, _ = _run_both(geometry, source, m, obs2, 500_000, 50_000, seed=5)
+    marg = Eg3.reshape(O3.shape, order='F').sum(axis=2)
+    assert marg.sum() == Eg3.sum()
+    assert_same_distribution(marg.reshape(-1, order='F'), 500_000, Eo2, 50_000, "3-variable marginal")
+
+
+def test_radiative_transfer_dropin_fields(gpu):
+    geometry, source, material, observation = cases.config1(n=150_000)
+    observation = dict(observation, time=np.arange(0, 1 + 1e-9, 0.05))
+    obs = solver.radiativeTransfer(geometry, source, material, observation, seed=3)
+    for k in ('d', 'acoustics', 'N', 't', 'Nt', 'x', 'Nx', 'dx', 'binX', 'y', 'Ny', 'dy', 'binY', 'z', 'Nz', 'dz',
+              'binZ', 'psi', 'Npsi', 'dpsi', 'binPsi', 'energyDensity', 'energy'):
+        assert k in obs, k
+    assert obs['N'] == 200_000                                   # packets of 1e5 (radiativeTransfer.m:23-28)
+    assert obs['energyDensity'].shape == (120, 1, obs['Nt'])
+    assert obs['energy'][0] == 0.0
+    np.testing.assert_allclose(obs['energy'][1:], 1.0, rtol=0, atol=0)
+    # integral of the density over the spherical shells recovers the energy
+    integ = (obs['energyDensity'][:, 0, :] * obs['dx'][:, None]).sum(axis=0) * obs['dy'][0] * obs['dz'][0]
+    np.testing.assert_allclose(integ, obs['energy'], rtol=1e-12)
+
+
+def test_forward_chunks_add_up(gpu):
+    """Two calls on disjoint particle ranges equal one call on the union (exactly:
+    the Philox stream of a particle depends only on its global id)."""
+    geometry, source, material, observation = cases.config1()
+    observation = dict(observation, time=np.arange(0, 0.6 + 1e-9, 0.02))
+    material = MaterialClass.prepareSigma(material, 3)
+    O = initializeObservation(geometry, True, observation, 1)
+    E1, _, _ = solver.forward_histogram(geometry, source, material, O, 300_000, seed=9, device='cuda')
+    E2, _, _ = solver.forward_histogram(geometry, source, material, O, 100_000, seed=9, device='cuda')
+    solver.forward_histogram(geometry, source, material, O, 200_000, seed=9, device='cuda', particle_offset=100_000,
+                             E=E2)
+    assert torch.equal(E1, E2)
