@@ -221,10 +221,10 @@ __device__ __forceinline__ int interface_plane(const MatDev& M, int b, const dou
 
 // Applies the reflection of a particle that crossed plane boundary B
 // (acoustic :184-187; elastic :190-321).
-__device__ __forceinline__ void plane_reflect(const MatDev& M, const rtmc_boundary& B, double x[3], double d[3],
-                                              double q[3], int& isP, double uA, double uB) {
+__device__ __forceinline__ void plane_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, double x[3],
+                                              double d[3], double q[3], int& isP, double uA, double uB) {
   int b = B.dir - 1;
-  if (M.acoustics) {
+  if (acoustic) {
     x[b] = (2 * B.val) - x[b];
     d[b] = -d[b];
     return;
@@ -284,13 +284,13 @@ __device__ __forceinline__ void reflect_dq(double d[3], double q[3], const doubl
 
 // Applies the reflection of a particle outside cylinder B
 // (propagateParticleSmallDt.m:344-578).
-__device__ __forceinline__ void cyl_reflect(const MatDev& M, const rtmc_boundary& B, double x[3], double d[3],
-                                            double q[3], int& isP, double uA, double uB) {
+__device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, double x[3],
+                                            double d[3], double q[3], int& isP, double uA, double uB) {
   double r = sqrt(x[0] * x[0] + x[1] * x[1]);
   double ov = r - B.val;
   double nv[3] = {x[0] / r, x[1] / r, 0.0};
   for (int k = 0; k < 3; ++k) x[k] = x[k] - 2 * ov * nv[k];  // :352
-  if (M.acoustics) {                                         // :354-359
+  if (acoustic) {                                            // :354-359
     double dp = dot3(d, nv);
     for (int k = 0; k < 3; ++k) d[k] = d[k] - 2 * dp * nv[k];
     return;
@@ -375,9 +375,9 @@ __device__ __forceinline__ void cyl_reflect(const MatDev& M, const rtmc_boundary
 // propagateParticle.m:33-72).  bit2d: the 2-D randi([0 1]) sign bit;
 // uphi: 3-D azimuth uniform; uth: polar-angle uniform; uconv: the P2P (P)
 // or S2S (S) uniform.  Returns theta.
-template <int DIM>
-__device__ __forceinline__ double scatter_angles(const MatDev& M, double d[3], double q[3], int& isP, bool bit2d,
-                                                 double uphi, double uth, double uconv) {
+template <int DIM, typename TabF>
+__device__ __forceinline__ double scatter_angles_t(const MatDev& M, bool acoustic, double d[3], double q[3], int& isP,
+                                                   bool bit2d, double uphi, double uth, double uconv, TabF lookup) {
   if (DIM == 2) {
     double s = bit2d ? -1.0 : 1.0;
     for (int k = 0; k < 3; ++k) q[k] = s * q[k];
@@ -389,9 +389,7 @@ __device__ __forceinline__ double scatter_angles(const MatDev& M, double d[3], d
     sincos(phi, &sp, &cp);
     for (int k = 0; k < 3; ++k) q[k] = cp * q[k] + sp * c[k];
   }
-  if (M.acoustics) {
-    return table_eval(M.inv[0].x, M.inv[0].y, M.inv[0].n, M.inv[0].extrap, uth);
-  }
+  if (acoustic) return lookup(0, uth);
   int from_p = isP;
   if (from_p) {
     if (uconv > M.P2P) isP = 0;
@@ -399,8 +397,16 @@ __device__ __forceinline__ double scatter_angles(const MatDev& M, double d[3], d
     if (uconv > M.S2S) isP = 1;
   }
   int tab = from_p ? (isP ? 0 : 1) : (isP ? 2 : 3);
-  const TableDev& T = M.inv[tab];
-  return table_eval(T.x, T.y, T.n, T.extrap, uth);
+  return lookup(tab, uth);
+}
+
+template <int DIM>
+__device__ __forceinline__ double scatter_angles(const MatDev& M, double d[3], double q[3], int& isP, bool bit2d,
+                                                 double uphi, double uth, double uconv) {
+  return scatter_angles_t<DIM>(M, M.acoustics != 0, d, q, isP, bit2d, uphi, uth, uconv, [&](int t, double u) {
+    const TableDev& T = M.inv[t];
+    return table_eval(T.x, T.y, T.n, T.extrap, u);
+  });
 }
 
 // dir <- cos(th) dir + sin(th) perp, perp <- -sin(th) dir + cos(th) perp,

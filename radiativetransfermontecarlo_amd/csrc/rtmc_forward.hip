@@ -27,6 +27,7 @@
 #include "rtmc_common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace rtmc {
 
@@ -54,6 +55,8 @@ struct FwdParams {
   double lamK[2];     // removal: omega dt_s / Q(mode) (per sub-step)
   int useQ;
   int edges_total;    // doubles of the LDS edge copy (0: edges read from global memory)
+  int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
+  int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
   int64_t hist_size;  // uint32 words of the LDS histogram (0 = global atomics)
 };
 
@@ -252,7 +255,7 @@ __device__ __forceinline__ void deposit(const FwdParams& p, const BinSetup& bs, 
     if (b < 0) return;
     idx += (int64_t)b * p.stride[k];
   }
-  if (!p.M.acoustics && !isP) idx += p.mode_stride;
+  if (!isP) idx += p.mode_stride;
   ndep += 1;
   if (LDSH) {
     atomicAdd(shist + idx, 1u);
@@ -261,13 +264,39 @@ __device__ __forceinline__ void deposit(const FwdParams& p, const BinSetup& bs, 
   }
 }
 
-template <int DIM, bool LDSH, bool QUANT>
+template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND>
 __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __restrict__ E,
                                                  unsigned long long* __restrict__ counters) {
   extern __shared__ double smem[];
   double* sedges = p.edges_total > 0 ? smem : nullptr;
-  uint32_t* shist = (uint32_t*)(smem + p.edges_total);
   __shared__ BinSetup s_bs;
+  __shared__ double s_tx0[4], s_tx1[4];
+  int* sguide = (int*)(smem + p.edges_total);
+  uint32_t* shist = (uint32_t*)(sguide + (p.guide_n > 0 ? p.ntab * (p.guide_n + 1) : 0));
+  if (p.guide_n > 0) {
+    // guide table: cell c holds the largest knot index k <= n-2 with x[k] <= c/G
+    // (Chen & Asau's guide for inversion of a tabulated CDF)
+    const int G = p.guide_n;
+    for (int t = 0; t < p.ntab; ++t) {
+      const TableDev& T = p.M.inv[t];
+      if (threadIdx.x == 0) {
+        s_tx0[t] = T.x[0];
+        s_tx1[t] = T.x[T.n - 1];
+      }
+      for (int c = threadIdx.x; c <= G; c += blockDim.x) {
+        double q = (double)c / (double)G;
+        int64_t lo = 0, hi = T.n - 1;
+        if (q >= T.x[0]) {
+          while (hi - lo > 1) {
+            int64_t mid = lo + ((hi - lo) >> 1);
+            if (T.x[mid] <= q) lo = mid;
+            else hi = mid;
+          }
+        }
+        sguide[t * (G + 1) + c] = (int)lo;
+      }
+    }
+  }
   if (p.edges_total > 0) {
     // gather the edge arrays into one LDS block
     for (int v = 0; v < 4; ++v) {
@@ -284,6 +313,19 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
   const BinSetup bs = s_bs;
 
   const MatDev& M = p.M;
+  // inverse-CDF lookup of the reference table (same interval and formula as
+  // table_eval), bracketed by the LDS guide when present
+  auto lookup = [&](int t, double u) -> double {
+    const TableDev& T = M.inv[t];
+    if (p.guide_n > 0 && u >= s_tx0[t] && u <= s_tx1[t]) {
+      const int G = p.guide_n;
+      int c = (int)(u * (double)G);
+      c = c < 0 ? 0 : (c > G - 1 ? G - 1 : c);
+      const int* g = sguide + t * (G + 1);
+      return table_eval_bracket(T.x, T.y, T.n, T.extrap, u, g[c], (int64_t)g[c + 1] + 1);
+    }
+    return table_eval(T.x, T.y, T.n, T.extrap, u);
+  };
   unsigned long long nscat = 0, ndep = 0, nbnd = 0, nkill = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride) {
@@ -298,7 +340,7 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
     rng.spare = 0.0;
     double x[3], d[3], q[3];
     birth_fwd<DIM>(p, rng, x, d, q);
-    int isP = (!M.acoustics && p.S.polarization == 1) ? 0 : 1;
+    int isP = (!AC && p.S.polarization == 1) ? 0 : 1;
 
     if (QUANT) {
       // ---- sub-stepped process, sampled event to event ----
@@ -312,9 +354,11 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
         const int64_t kobs = (int64_t)it * p.nsub;
         while (true) {
           int64_t kbd = kInf;
-          for (int ib = 0; ib < p.G.n_bnd; ++ib) {
-            int64_t nb = predict_boundary(p.G.bnd[ib], xb, d, h);
-            if (nb < kInf) kbd = min(kbd, kb + nb);
+          if (BND) {
+            for (int ib = 0; ib < p.G.n_bnd; ++ib) {
+              int64_t nb = predict_boundary(p.G.bnd[ib], xb, d, h);
+              if (nb < kInf) kbd = min(kbd, kb + nb);
+            }
           }
           int64_t j = min(Ks, min(Kk, kbd));
           if (j > kobs) break;
@@ -330,7 +374,7 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
           double x1[3] = {x[0], x[1], x[2]};
           for (int k = 0; k < 3; ++k) x[k] = x[k] + h * d[k];
           const int wasP = isP;
-          for (int ib = 0; ib < p.G.n_bnd; ++ib) {
+          for (int ib = 0; BND && ib < p.G.n_bnd; ++ib) {
             const rtmc_boundary& B = p.G.bnd[ib];
             bool hit = (B.dir < 4) ? plane_crossed(B, x, x1) : cyl_outside(B, x);
             if (!hit) continue;
@@ -341,15 +385,15 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
               break;
             }
             double uA = 0.0, uB = 0.0;
-            if (!M.acoustics) {
+            if (!AC) {
               uA = rng.next();
               uB = rng.next();
             }
-            if (B.dir < 4) plane_reflect(M, B, x, d, q, isP, uA, uB);
-            else cyl_reflect(M, B, x, d, q, isP, uA, uB);
+            if (B.dir < 4) plane_reflect(M, AC, B, x, d, q, isP, uA, uB);
+            else cyl_reflect(M, AC, B, x, d, q, isP, uA, uB);
           }
           if (!alive) break;
-          if (isP != wasP) {
+          if (!AC && isP != wasP) {
             // mode changed at the boundary: the scattering test of this sub-step
             // and the removal tests from the next one use the new mode
             h = (isP ? M.vp : M.vs) * p.dt_sub;
@@ -360,11 +404,11 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
             const int before = isP;
             double uphi = rng.next();
             double uth = rng.next();
-            double uconv = M.acoustics ? 0.0 : rng.next();
-            double th = scatter_angles<DIM>(M, d, q, isP, uphi < 0.5, uphi, uth, uconv);
+            double uconv = AC ? 0.0 : rng.next();
+            double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
             rotate_renorm(d, q, th);
             nscat += 1;
-            if (isP != before) {
+            if (!AC && isP != before) {
               h = (isP ? M.vp : M.vs) * p.dt_sub;
               if (p.useQ) Kk = j + geom_after(rng, p.lamK[isP ? 0 : 1]);
             }
@@ -396,8 +440,8 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
           tb = ts;
           double uphi = rng.next();
           double uth = rng.next();
-          double uconv = M.acoustics ? 0.0 : rng.next();
-          double th = scatter_angles<DIM>(M, d, q, isP, uphi < 0.5, uphi, uth, uconv);
+          double uconv = AC ? 0.0 : rng.next();
+          double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
           rotate_renorm(d, q, th);
           nscat += 1;
           v = isP ? M.vp : M.vs;
@@ -439,15 +483,32 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
 // ------------------------------------------------------------------
 // host side
 
-template <int DIM, bool LDSH, bool QUANT>
+template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND>
 static int launch_fwd(const FwdParams& p, size_t smem, unsigned grid, unsigned block, uint32_t* E,
                       unsigned long long* counters, hipStream_t st) {
-  auto kern = k_forward<DIM, LDSH, QUANT>;
+  auto kern = k_forward<DIM, LDSH, QUANT, AC, BND>;
   if (smem > 64 * 1024) RTMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)smem));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), smem, st, p, E, counters);
   RTMC_HIP(hipGetLastError());
   return RTMC_OK;
+}
+
+// compile-time specialisation: dimension, histogram placement, process,
+// acoustic/elastic, with/without boundaries (the event-driven process has none)
+template <int DIM, bool LDSH>
+static int dispatch_fwd(const FwdParams& p, bool quant, bool ac, bool bnd, size_t smem, unsigned grid,
+                        unsigned block, uint32_t* E, unsigned long long* counters, hipStream_t st) {
+  if (quant) {
+    if (ac) {
+      if (bnd) return launch_fwd<DIM, LDSH, true, true, true>(p, smem, grid, block, E, counters, st);
+      return launch_fwd<DIM, LDSH, true, true, false>(p, smem, grid, block, E, counters, st);
+    }
+    if (bnd) return launch_fwd<DIM, LDSH, true, false, true>(p, smem, grid, block, E, counters, st);
+    return launch_fwd<DIM, LDSH, true, false, false>(p, smem, grid, block, E, counters, st);
+  }
+  if (ac) return launch_fwd<DIM, LDSH, false, true, false>(p, smem, grid, block, E, counters, st);
+  return launch_fwd<DIM, LDSH, false, false, false>(p, smem, grid, block, E, counters, st);
 }
 
 }  // namespace rtmc
@@ -539,6 +600,12 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
   p.edges_total = edges_lds ? total : 0;
   if (!edges_lds) edge_bytes = 0;
   size_t hist_bytes = (size_t)hsize * sizeof(uint32_t);
+  p.ntab = m.acoustics ? 1 : 4;
+  int guide = 2048;
+  if (const char* g = getenv("RTMC_GUIDE_CELLS")) guide = atoi(g);
+  RTMC_REQUIRE(guide >= 0 && guide <= (1 << 16) && (guide & (guide - 1)) == 0,
+               "RTMC_GUIDE_CELLS must be 0 or a power of two <= 65536");
+  auto guide_bytes = [&](int G) { return G > 0 ? (size_t)p.ntab * (size_t)(G + 1) * sizeof(int) : (size_t)0; };
   bool ldsh;
   if (prob->hist_mode == 1) ldsh = false;
   else if (prob->hist_mode == 2) ldsh = true;
@@ -547,8 +614,12 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
     set_error("LDS histogram of %zu bytes exceeds the %d bytes of LDS per work-group", hist_bytes, max_smem);
     return RTMC_ERR_ARG;
   }
+  // the guide takes what the histogram leaves
+  while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
+  if (guide < 64) guide = 0;
+  p.guide_n = guide;
   p.hist_size = ldsh ? hsize : 0;
-  size_t smem = edge_bytes + (ldsh ? hist_bytes : 0);
+  size_t smem = edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0);
 
   unsigned block = prob->block_size > 0 ? (unsigned)prob->block_size : 256u;
   RTMC_REQUIRE(block % 64 == 0 && block <= 256, "block_size must be a multiple of 64 and <= 256");
@@ -563,23 +634,13 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
   unsigned grid = (unsigned)std::max<int64_t>(1, std::min(want, cap));
 
   const bool quant = prob->time_steps == 0;
-#define RTMC_DISPATCH(D, L, Q) return launch_fwd<D, L, Q>(p, smem, grid, block, E, (unsigned long long*)counters, st)
+  const bool ac = m.acoustics != 0;
+  const bool bnd = prob->geometry.n_bnd > 0;
+  unsigned long long* cnt = (unsigned long long*)counters;
   if (dim == 2) {
-    if (ldsh) {
-      if (quant) RTMC_DISPATCH(2, true, true);
-      else RTMC_DISPATCH(2, true, false);
-    } else {
-      if (quant) RTMC_DISPATCH(2, false, true);
-      else RTMC_DISPATCH(2, false, false);
-    }
-  } else {
-    if (ldsh) {
-      if (quant) RTMC_DISPATCH(3, true, true);
-      else RTMC_DISPATCH(3, true, false);
-    } else {
-      if (quant) RTMC_DISPATCH(3, false, true);
-      else RTMC_DISPATCH(3, false, false);
-    }
+    if (ldsh) return dispatch_fwd<2, true>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
+    return dispatch_fwd<2, false>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
   }
-#undef RTMC_DISPATCH
+  if (ldsh) return dispatch_fwd<3, true>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
+  return dispatch_fwd<3, false>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
 }

@@ -159,8 +159,8 @@ __global__ __launch_bounds__(kBlock) void k_smalldt(const MatDev M, const rtmc_g
         uA = D(i, 6 + 2 * ib, i1);
         uB = D(i, 7 + 2 * ib, i1);
       }
-      if (B.dir < 4) plane_reflect(M, B, x, d, q, isP, uA, uB);
-      else cyl_reflect(M, B, x, d, q, isP, uA, uB);
+      if (B.dir < 4) plane_reflect(M, M.acoustics != 0, B, x, d, q, isP, uA, uB);
+      else cyl_reflect(M, M.acoustics != 0, B, x, d, q, isP, uA, uB);
     }
     if (!alive) break;
     double lam = dt / (isP ? M.mftP : M.mftS);  // :591-601

@@ -144,3 +144,28 @@ def test_forward_chunks_add_up(gpu):
     solver.forward_histogram(geometry, source, material, O, 200_000, seed=9, device='cuda', particle_offset=100_000,
                              E=E2)
     assert torch.equal(E1, E2)
+
+
+def test_paasschens_2d_exact_solution(gpu):
+    """Event-driven 2-D isotropic acoustic scattering against the exact 2-D
+    solution of Paasschens (1997) (+Comparison/analyticalPaasschens.m:65-68),
+    inside the ballistic front, within 3 % (bin averaging + Monte Carlo error)."""
+    from tests.analytic import paasschens_2d_diffuse
+    geometry = {'dimension': 2, 'frame': 'spherical'}
+    source = {'lambda': 1e-4}
+    observation = {'x': np.arange(0, 10 + 1e-9, 0.1), 'time': np.arange(0, 4 + 1e-9, 0.1)}
+    m = MaterialClass.prepareSigma(cases.iso_acoustic(2, v=2.0), 2)
+    m.timeSteps = 1
+    N = 10_000_000
+    O = initializeObservation(geometry, True, observation, N)
+    E, _, _ = solver.forward_histogram(geometry, source, m, O, N, seed=17, device='cuda', time_steps=1)
+    from radiativetransfermontecarlo_amd.observation import normalize
+    dens, energy = normalize(O, E.cpu().numpy().view(np.uint32), N)
+    np.testing.assert_array_equal(energy[1:], 1.0)
+    r = O.obs['x']
+    for it in (20, 30, 40):
+        t = O.t[it]
+        ref = paasschens_2d_diffuse(r, t, m.Sigma, 2.0)
+        sel = (r > 0.5) & (r < 0.7 * 2.0 * t)
+        rel = dens[sel, 0, it] / ref[sel] - 1
+        assert np.max(np.abs(rel)) < 0.03, (t, np.max(np.abs(rel)))

@@ -1,10 +1,14 @@
 """GPU parity of the function-level kernels against the CPU oracle.
 
 REPLAY: the oracle and the kernel read the same recorded variates; the
-histogram counts must agree exactly, the particle state to 1e-12 (the two
-sides use different libm implementations of sin/cos/acos, which may differ
-in the last bit).  PHILOX: both sides generate the same counter-based
-variates; same bar.
+histogram counts must agree exactly after every call; times to 1e-15
+relative, positions (relative to the domain size), direction and
+polarisation vectors to 1e-9.  The two
+sides use different libm implementations of sin/cos/acos (glibc / ROCm
+device libraries), which may differ in the last bit; the renormalisation of
+a nearly parallel cross product at close-to-normal boundary incidence
+(propagateParticleSmallDt.m:444-453) amplifies such a difference by up to
+1e4.  PHILOX: both sides generate the same counter-based variates; same bar.
 """
 import math
 
@@ -20,7 +24,7 @@ from tests import cases
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-12
+TOL = 1e-9
 
 
 def _to_gpu(H: orc.HostParticles, dev) -> solver.Particles:
@@ -39,6 +43,25 @@ def _compare(G: solver.Particles, H: orc.HostParticles, tol=TOL):
     a = H.alive.astype(bool)
     np.testing.assert_allclose(g['dir'][:, a], H.dir[:, a], rtol=0, atol=tol)
     np.testing.assert_allclose(g['perp'][:, a], H.perp[:, a], rtol=0, atol=tol)
+    # the bulk agrees to the last bits; only ill-conditioned boundary cases reach the bound
+    assert np.median(np.abs(g['x'] - H.x)) <= 1e-15 * max(1.0, np.abs(H.x).max())
+
+
+def _compare_counts(geo, acoustics, G, H, dev):
+    """observeTime on both sides after a propagation call: counts bit-exact."""
+    d = geo['dimension']
+    geo = dict(geo, frame='cartesian')
+    e = np.linspace(-2.5, 2.5, 41)
+    obs = {'x': e, 'y': e, 'time': [0, 1]}
+    if d == 3:
+        obs['z'] = np.linspace(-2.5, 2.5, 5)
+    O = initializeObservation(geo, acoustics, obs, H.N, allow_nd=True)
+    g = solver.build_geometry({'dimension': d, 'frame': 'cartesian'})
+    Eh = np.zeros(O.size, dtype=np.uint32)
+    orc.observe(g, orc.HostBins(O), acoustics, H, Eh, 1)
+    Eg = torch.zeros(O.size, dtype=torch.int32, device=dev)
+    solver.observeTime(g, solver.DeviceBins(O, dev), acoustics, G, Eg, 1)
+    np.testing.assert_array_equal(Eg.cpu().numpy().view(np.uint32), Eh)
 
 
 def _replay_values(rng, steps, slots, N, d, scat_p=0.3, bit_slots=()):
@@ -144,6 +167,7 @@ def test_propagate_smalldt(gpu, mkind, d, bnd, Q, kind):
         assert ih.n_sub == ig.n_sub and ih.dt == ig.dt
         torch.cuda.synchronize()
         _compare(G, H)
+        _compare_counts(geo, acoustics, G, H, gpu)
     if bnd and any(b.get('type') == 'absorbing' for b in bnd):
         assert H.alive.mean() < 1.0
 
@@ -193,6 +217,7 @@ def test_propagate_events(gpu, mkind, d, kind):
         solver.propagateParticle(dm, G, T, gd)
         torch.cuda.synchronize()
         _compare(G, H)
+        _compare_counts({'dimension': d}, acoustics, G, H, gpu)
 
 
 OBS = [
@@ -200,7 +225,7 @@ OBS = [
     (3, 'spherical', {'x': np.linspace(0, 1.5, 13), 'y': np.linspace(-np.pi, np.pi, 7)}),
     (3, 'spherical', {'x': np.linspace(0, 1.5, 13), 'z': np.linspace(-np.pi / 2, np.pi / 2, 5)}),
     (3, 'cylindrical', {'x': np.linspace(0, 1.5, 13), 'z': np.linspace(-1, 1, 9)}),
-    (3, 'cylindrical', {'y': np.linspace(-np.pi, np.pi, 9), 'directions': np.linspace(0, np.pi, 4)}),
+    (3, 'cylindrical', {'x': [0, 1.2], 'y': np.linspace(-np.pi, np.pi, 9), 'directions': np.linspace(0, np.pi, 4)}),
     (3, 'cartesian', {'x': np.linspace(-1, 1, 9), 'y': np.linspace(-1, 1, 9), 'z': [-0.5, 0.5]}),
     (3, 'cartesian', {'x': [-2, 2], 'z': np.linspace(-1, 1, 9), 'directions': np.linspace(0, np.pi, 7),
                       'y': [-0.4, 0.6]}),
