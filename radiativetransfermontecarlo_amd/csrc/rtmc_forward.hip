@@ -55,6 +55,9 @@ struct FwdParams {
   double lamK[2];     // removal: omega dt_s / Q(mode) (per sub-step)
   int useQ;
   int edges_total;    // doubles of the LDS edge copy (0: edges read from global memory)
+  unsigned long long* work;  // global particle counter of the launch (zeroed)
+  int chunk;          // particles fetched per counter access
+  int batch;          // lanes waiting for an event before a wavefront takes an event step
   int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
   int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
   int64_t hist_size;  // uint32 words of the LDS histogram (0 = global atomics)
@@ -327,115 +330,184 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
     return table_eval(T.x, T.y, T.n, T.extrap, u);
   };
   unsigned long long nscat = 0, ndep = 0, nbnd = 0, nkill = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride) {
-    uint64_t pid = p.pid0 + (uint64_t)i;
-    Rng rng;
-    rng.k0 = p.k0;
-    rng.k1 = p.k1;
-    rng.plo = (uint32_t)pid;
-    rng.phi = (uint32_t)(pid >> 32);
-    rng.c = 0u;
-    rng.has = false;
-    rng.spare = 0.0;
-    double x[3], d[3], q[3];
-    birth_fwd<DIM>(p, rng, x, d, q);
-    int isP = (!AC && p.S.polarization == 1) ? 0 : 1;
+  // ---- lane state machine ------------------------------------------------
+  // Each lane holds one particle and its own next observation index.  One
+  // iteration is either a DEPOSIT step (every flying lane deposits its next
+  // observation) or an EVENT step (lanes whose next event comes before that
+  // observation process it, idle lanes take a new particle).  The wavefront
+  // takes an event step only when at least `batch` lanes wait for one (or no
+  // lane can fly), so the expensive event path runs for many lanes at once
+  // instead of once per observation for whichever lane happens to need it.
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  Rng rng;
+  rng.k0 = p.k0;
+  rng.k1 = p.k1;
+  rng.plo = 0u;
+  rng.phi = 0u;
+  rng.c = 0u;
+  rng.has = false;
+  rng.spare = 0.0;
+  double xb[3] = {0.0, 0.0, 0.0}, d[3] = {0.0, 0.0, 1.0}, q[3] = {1.0, 0.0, 0.0};
+  int isP = 1;
+  double h = 0.0;                 // sub-stepped: step length v dt_s; event-driven: speed v
+  int64_t kb = 0, Ks = kInf, Kk = kInf, kev = kInf;   // sub-stepped: base, next scatter / removal / event
+  double tb = 0.0, ts = INFINITY;                     // event-driven: base time, next scattering time
+  int it = 1;                     // next observation index
+  bool has = false;               // lane holds a particle
+  bool need = true;               // lane wants a particle
+  // wave-uniform pool of particle indices fetched from the global counter
+  int64_t pool_next = 0, pool_end = 0;
+  bool exhausted = false;
 
-    if (QUANT) {
-      // ---- sub-stepped process, sampled event to event ----
-      double h = (isP ? M.vp : M.vs) * p.dt_sub;
-      double xb[3] = {x[0], x[1], x[2]};
-      int64_t kb = 0;
-      int64_t Ks = geom_after(rng, p.lamS[isP ? 0 : 1]);
-      int64_t Kk = p.useQ ? geom_after(rng, p.lamK[isP ? 0 : 1]) : kInf;
-      bool alive = true;
-      for (int it = 1; it < p.n_times && alive; ++it) {
-        const int64_t kobs = (int64_t)it * p.nsub;
-        while (true) {
-          int64_t kbd = kInf;
-          if (BND) {
-            for (int ib = 0; ib < p.G.n_bnd; ++ib) {
-              int64_t nb = predict_boundary(p.G.bnd[ib], xb, d, h);
-              if (nb < kInf) kbd = min(kbd, kb + nb);
-            }
+  auto next_event = [&]() {
+    // sub-stepped: next sub-step at which something happens to this particle
+    int64_t kbd = kInf;
+    if (BND) {
+      for (int ib = 0; ib < p.G.n_bnd; ++ib) {
+        int64_t nb = predict_boundary(p.G.bnd[ib], xb, d, h);
+        if (nb < kInf) kbd = min(kbd, kb + nb);
+      }
+    }
+    kev = min(Ks, min(Kk, kbd));
+  };
+
+  while (true) {
+    const bool pend_ev = has && (QUANT ? (kev <= (int64_t)it * p.nsub) : (ts <= (double)it * p.dt_obs));
+    const bool pend = need || pend_ev;
+    const bool fly = has && !pend_ev;
+    const uint64_t pm = __ballot(pend);
+    const uint64_t fm = __ballot(fly);
+    if (pm == 0ull && fm == 0ull) break;
+    if (__popcll(pm) >= p.batch || fm == 0ull) {
+      // ---------------- EVENT step ----------------
+      const uint64_t nm = __ballot(need);
+      if (nm) {
+        // hand out particle indices to the lanes that need one
+        const int n = __popcll(nm);
+        int64_t L = pool_end - pool_next;
+        int64_t nb0 = 0, nb1 = 0;
+        if (L < n && !exhausted) {
+          unsigned long long base = 0;
+          if (lane == __ffsll((long long)nm) - 1) base = atomicAdd(p.work, (unsigned long long)p.chunk);
+          base = __shfl(base, __ffsll((long long)nm) - 1, 64);
+          if ((int64_t)base >= p.n) {
+            exhausted = true;
+          } else {
+            nb0 = (int64_t)base;
+            nb1 = min((int64_t)base + (int64_t)p.chunk, p.n);
           }
-          int64_t j = min(Ks, min(Kk, kbd));
-          if (j > kobs) break;
-          // straight flight to the end of sub-step j-1
-          double m = (double)(j - 1 - kb);
+        }
+        if (need) {
+          const int r = __popcll(nm & lt_mask);
+          int64_t id = -1;
+          if (r < L) id = pool_next + r;
+          else if (r - L < nb1 - nb0) id = nb0 + (r - L);
+          if (id >= 0) {
+            const uint64_t pid = p.pid0 + (uint64_t)id;
+            rng.plo = (uint32_t)pid;
+            rng.phi = (uint32_t)(pid >> 32);
+            rng.c = 0u;
+            rng.has = false;
+            birth_fwd<DIM>(p, rng, xb, d, q);
+            isP = (!AC && p.S.polarization == 1) ? 0 : 1;
+            it = 1;
+            has = true;
+            need = false;
+            if (QUANT) {
+              h = (isP ? M.vp : M.vs) * p.dt_sub;
+              kb = 0;
+              Ks = geom_after(rng, p.lamS[isP ? 0 : 1]);
+              Kk = p.useQ ? geom_after(rng, p.lamK[isP ? 0 : 1]) : kInf;
+              next_event();
+            } else {
+              h = isP ? M.vp : M.vs;
+              tb = 0.0;
+              ts = -log(rng.next()) * (isP ? M.mftP : M.mftS);
+            }
+          } else if (exhausted || r - L >= nb1 - nb0) {
+            need = false;  // nothing left to trace
+          }
+        }
+        if (n <= L) {
+          pool_next += n;
+        } else if (nb1 > nb0) {
+          pool_next = nb0 + min((int64_t)(n - L), nb1 - nb0);
+          pool_end = nb1;
+        } else {
+          pool_next = pool_end;
+        }
+      }
+      if (pend_ev) {
+        if (QUANT) {
+          // sub-step j = kev in full (propagateParticleSmallDt.m:85-642); straight
+          // flight from the base to the end of sub-step j-1 first
+          const int64_t j = kev;
+          const double m = (double)(j - 1 - kb);
+          double x[3];
           for (int k = 0; k < 3; ++k) x[k] = xb[k] + (m * h) * d[k];
-          if (Kk == j) {  // removed at the start of sub-step j (propagateParticleSmallDt.m:100-138)
+          bool alive = true;
+          if (Kk == j) {  // removed at the start of sub-step j (:100-138)
             alive = false;
             nkill += 1;
-            break;
-          }
-          // sub-step j in full: move (:141-143), boundaries (:146-582), scattering (:591-639)
-          double x1[3] = {x[0], x[1], x[2]};
-          for (int k = 0; k < 3; ++k) x[k] = x[k] + h * d[k];
-          const int wasP = isP;
-          for (int ib = 0; BND && ib < p.G.n_bnd; ++ib) {
-            const rtmc_boundary& B = p.G.bnd[ib];
-            bool hit = (B.dir < 4) ? plane_crossed(B, x, x1) : cyl_outside(B, x);
-            if (!hit) continue;
-            nbnd += 1;
-            if (B.type == RTMC_BND_ABSORBING) {
-              alive = false;
-              nkill += 1;
-              break;
+          } else {
+            double x1[3] = {x[0], x[1], x[2]};
+            for (int k = 0; k < 3; ++k) x[k] = x[k] + h * d[k];
+            const int wasP = isP;
+            for (int ib = 0; BND && ib < p.G.n_bnd; ++ib) {
+              const rtmc_boundary& B = p.G.bnd[ib];
+              bool hit = (B.dir < 4) ? plane_crossed(B, x, x1) : cyl_outside(B, x);
+              if (!hit) continue;
+              nbnd += 1;
+              if (B.type == RTMC_BND_ABSORBING) {
+                alive = false;
+                nkill += 1;
+                break;
+              }
+              double uA = 0.0, uB = 0.0;
+              if (!AC) {
+                uA = rng.next();
+                uB = rng.next();
+              }
+              if (B.dir < 4) plane_reflect(M, AC, B, x, d, q, isP, uA, uB);
+              else cyl_reflect(M, AC, B, x, d, q, isP, uA, uB);
             }
-            double uA = 0.0, uB = 0.0;
-            if (!AC) {
-              uA = rng.next();
-              uB = rng.next();
+            if (alive) {
+              if (!AC && isP != wasP) {
+                // mode changed at the boundary: this sub-step's scattering test and
+                // the removal tests from the next one use the new mode
+                h = (isP ? M.vp : M.vs) * p.dt_sub;
+                Ks = j - 1 + geom_after(rng, p.lamS[isP ? 0 : 1]);
+                Kk = p.useQ ? j + geom_after(rng, p.lamK[isP ? 0 : 1]) : kInf;
+              }
+              if (Ks == j) {
+                const int before = isP;
+                double uphi = rng.next();
+                double uth = rng.next();
+                double uconv = AC ? 0.0 : rng.next();
+                double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
+                rotate_renorm(d, q, th);
+                nscat += 1;
+                if (!AC && isP != before) {
+                  h = (isP ? M.vp : M.vs) * p.dt_sub;
+                  if (p.useQ) Kk = j + geom_after(rng, p.lamK[isP ? 0 : 1]);
+                }
+                Ks = j + geom_after(rng, p.lamS[isP ? 0 : 1]);
+              }
+              xb[0] = x[0];
+              xb[1] = x[1];
+              xb[2] = x[2];
+              kb = j;
+              next_event();
             }
-            if (B.dir < 4) plane_reflect(M, AC, B, x, d, q, isP, uA, uB);
-            else cyl_reflect(M, AC, B, x, d, q, isP, uA, uB);
           }
-          if (!alive) break;
-          if (!AC && isP != wasP) {
-            // mode changed at the boundary: the scattering test of this sub-step
-            // and the removal tests from the next one use the new mode
-            h = (isP ? M.vp : M.vs) * p.dt_sub;
-            Ks = j - 1 + geom_after(rng, p.lamS[isP ? 0 : 1]);
-            Kk = p.useQ ? j + geom_after(rng, p.lamK[isP ? 0 : 1]) : kInf;
+          if (!alive) {
+            has = false;
+            need = true;
           }
-          if (Ks == j) {
-            const int before = isP;
-            double uphi = rng.next();
-            double uth = rng.next();
-            double uconv = AC ? 0.0 : rng.next();
-            double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
-            rotate_renorm(d, q, th);
-            nscat += 1;
-            if (!AC && isP != before) {
-              h = (isP ? M.vp : M.vs) * p.dt_sub;
-              if (p.useQ) Kk = j + geom_after(rng, p.lamK[isP ? 0 : 1]);
-            }
-            Ks = j + geom_after(rng, p.lamS[isP ? 0 : 1]);
-          }
-          xb[0] = x[0];
-          xb[1] = x[1];
-          xb[2] = x[2];
-          kb = j;
-        }
-        if (!alive) break;
-        double m = (double)(kobs - kb);
-        double X[3];
-        for (int k = 0; k < 3; ++k) X[k] = xb[k] + (m * h) * d[k];
-        deposit<DIM, LDSH>(p, bs, sedges, shist, E, X, d[2], isP, it, ndep);
-      }
-    } else {
-      // ---- event-driven process (propagateParticle.m) ----
-      double v = isP ? M.vp : M.vs;
-      double tb = 0.0;
-      double xb[3] = {x[0], x[1], x[2]};
-      double mft = isP ? M.mftP : M.mftS;
-      double ts = tb - log(rng.next()) * mft;
-      for (int it = 1; it < p.n_times; ++it) {
-        const double tobs = (double)it * p.dt_obs;
-        while (ts <= tobs) {
-          double f = v * (ts - tb);
+        } else {
+          // event-driven: scattering at time ts (propagateParticle.m:24-72)
+          const double f = h * (ts - tb);
           for (int k = 0; k < 3; ++k) xb[k] = xb[k] + f * d[k];
           tb = ts;
           double uphi = rng.next();
@@ -444,14 +516,27 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
           double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
           rotate_renorm(d, q, th);
           nscat += 1;
-          v = isP ? M.vp : M.vs;
-          mft = isP ? M.mftP : M.mftS;
-          ts = tb - log(rng.next()) * mft;
+          h = isP ? M.vp : M.vs;
+          ts = tb - log(rng.next()) * (isP ? M.mftP : M.mftS);
         }
-        double f = v * (tobs - tb);
+      }
+    } else {
+      // ---------------- DEPOSIT step ----------------
+      if (fly) {
         double X[3];
-        for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
+        if (QUANT) {
+          const double m = (double)((int64_t)it * p.nsub - kb);
+          for (int k = 0; k < 3; ++k) X[k] = xb[k] + (m * h) * d[k];
+        } else {
+          const double f = h * ((double)it * p.dt_obs - tb);
+          for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
+        }
         deposit<DIM, LDSH>(p, bs, sedges, shist, E, X, d[2], isP, it, ndep);
+        it += 1;
+        if (it >= p.n_times) {
+          has = false;
+          need = true;
+        }
       }
     }
   }
@@ -514,6 +599,9 @@ static int dispatch_fwd(const FwdParams& p, bool quant, bool ac, bool bnd, size_
 }  // namespace rtmc
 
 using namespace rtmc;
+
+static int launch_all(const FwdParams& p, int dim, bool ldsh, bool quant, bool ac, bool bnd, size_t smem,
+                      unsigned grid, unsigned block, uint32_t* E, unsigned long long* cnt, hipStream_t st);
 
 extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* counters, void* stream) {
   clear_error();
@@ -633,10 +721,27 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
   int64_t cap = (int64_t)ncu * per_cu;
   unsigned grid = (unsigned)std::max<int64_t>(1, std::min(want, cap));
 
-  const bool quant = prob->time_steps == 0;
-  const bool ac = m.acoustics != 0;
-  const bool bnd = prob->geometry.n_bnd > 0;
-  unsigned long long* cnt = (unsigned long long*)counters;
+  // dynamic work distribution: one zeroed counter per launch
+  p.chunk = 256;
+  p.batch = 24;
+  if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
+  RTMC_REQUIRE(p.batch >= 1 && p.batch <= 64, "RTMC_EVENT_BATCH must be in [1, 64]");
+  void* work = nullptr;
+  RTMC_HIP(hipMallocAsync(&work, sizeof(unsigned long long), st));
+  RTMC_HIP(hipMemsetAsync(work, 0, sizeof(unsigned long long), st));
+  p.work = (unsigned long long*)work;
+  int rc = launch_all(p, dim, ldsh, prob->time_steps == 0, m.acoustics != 0, prob->geometry.n_bnd > 0, smem, grid,
+                      block, E, (unsigned long long*)counters, st);
+  hipError_t fe = hipFreeAsync(work, st);
+  if (rc == RTMC_OK && fe != hipSuccess) {
+    set_error("hipFreeAsync failed: %s", hipGetErrorString(fe));
+    return RTMC_ERR_HIP;
+  }
+  return rc;
+}
+
+static int launch_all(const FwdParams& p, int dim, bool ldsh, bool quant, bool ac, bool bnd, size_t smem,
+                      unsigned grid, unsigned block, uint32_t* E, unsigned long long* cnt, hipStream_t st) {
   if (dim == 2) {
     if (ldsh) return dispatch_fwd<2, true>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
     return dispatch_fwd<2, false>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
