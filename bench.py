@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--ppt", type=int, default=0)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="flight arithmetic of the fused kernel (fp32: FP32 flight, FP64 time)")
     return ap.parse_args()
 
 
@@ -100,7 +102,8 @@ def main():
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         keep = solver.forward_histogram(geometry, source, material, O, N, seed=seed, device=dev, E=E, counters=cnt,
-                                        dmat=dmat, block_size=args.block, particles_per_thread=args.ppt)
+                                        dmat=dmat, block_size=args.block, particles_per_thread=args.ppt,
+                                        precision=args.precision)
         ev1.record(stream)
         return ev0, ev1, keep
 
@@ -134,7 +137,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 flight / f64 time",
         "data": "synthetic (Philox4x32-10 particles; material tables from the reference's formulas)",
         "config": {"workload": "2-D elastic P-SV, cartesian, 1e7 particles, Q=500, x,y=-50:0.5:50, 36 direction "
                                "bins, t=0:0.05:20 (config 2)", "particles": N, "histogram_bytes": int(O.size) * 4,

@@ -73,18 +73,27 @@ struct SlotDraws {
 };
 
 // ------------------------------------------------------------------
-// Small vector helpers.  Sums are left to right, as MATLAB's sum(.,2).
-__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
+// Small vector helpers, generic in the real type (double: the reference's
+// arithmetic; float: the FP32 free-flight variant).  Sums are left to right,
+// as MATLAB's sum(.,2).
+template <typename T>
+__device__ __forceinline__ void cross3(const T a[3], const T b[3], T c[3]) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
-__device__ __forceinline__ double sumsq3(const double a[3]) { return (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]; }
-__device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
+template <typename T>
+__device__ __forceinline__ T sumsq3(const T a[3]) { return (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]; }
+template <typename T>
+__device__ __forceinline__ T dot3(const T a[3], const T b[3]) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
-__device__ __forceinline__ double sgn(double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); }
+template <typename T>
+__device__ __forceinline__ T sgn(T v) { return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0)); }
 __device__ __forceinline__ double acosd(double c) { return acos(c) * 180.0 / kPi; }
+__device__ __forceinline__ double acosd(float c) { return (double)(acosf(c) * (180.0f / 3.14159265f)); }
+__device__ __forceinline__ void sc(double a, double* s, double* c) { sincos(a, s, c); }
+__device__ __forceinline__ void sc(float a, float* s, float* c) { sincosf(a, s, c); }
 
 // ------------------------------------------------------------------
 // griddedInterpolant(x, y)(q): linear inside, linear or nearest outside
@@ -167,104 +176,110 @@ struct MatDev {
 // ------------------------------------------------------------------
 // Boundary interaction of one particle, given that it crossed boundary B
 // during the current sub-step (propagateParticleSmallDt.m:167-581).
-// Returns true if the particle was absorbed.  uA, uB: the two boundary
-// variates (P: reflect-or-convert; S: SV/SH split; SV: reflect-or-convert).
-__device__ __forceinline__ bool plane_crossed(const rtmc_boundary& B, const double x[3], const double x1[3]) {
+// uA, uB: the two boundary variates (P: reflect-or-convert; S: SV/SH split;
+// SV: reflect-or-convert).
+template <typename T>
+__device__ __forceinline__ bool plane_crossed(const rtmc_boundary& B, const T x[3], const T x1[3]) {
   int b = B.dir - 1;
-  return (x[b] - B.val) * (x1[b] - B.val) < 0;  // :171-172
+  const T v = (T)B.val;
+  return (x[b] - v) * (x1[b] - v) < T(0);  // :171-172
 }
 
-__device__ __forceinline__ bool cyl_outside(const rtmc_boundary& B, const double x[3]) {
-  double r = sqrt(x[0] * x[0] + x[1] * x[1]);  // :328-329
-  return (r - B.val) > 0;                      // :332-334
+template <typename T>
+__device__ __forceinline__ bool cyl_outside(const rtmc_boundary& B, const T x[3]) {
+  T r = sqrt(x[0] * x[0] + x[1] * x[1]);  // :328-329
+  return (r - (T)B.val) > T(0);            // :332-334
 }
 
 // MaterialClass.MaterialInterface branch at plane of normal e_b
 // (MaterialClass.m:1587-1735): 0 P->P, 1 P->SV, 2 SH, 3 SV->SV, 4 SV->P.
-__device__ __forceinline__ int interface_plane(const MatDev& M, int b, const double d[3], const double q[3],
-                                               int isP, double uA, double uB) {
-  double nv[3] = {0.0, 0.0, 0.0};
-  nv[b] = 1.0;
+template <typename T>
+__device__ __forceinline__ int interface_plane(const MatDev& M, int b, const T d[3], const T q[3], int isP, double uA,
+                                               double uB) {
+  T nv[3] = {T(0), T(0), T(0)};
+  nv[b] = T(1);
   if (isP) {
-    double a = fabs(dot3(d, nv));  // :1597-1599
-    if (a > 1) a = 1;
-    double inc = acosd(a);                                               // :1600
+    T a = fabs(dot3(d, nv));  // :1597-1599
+    if (a > T(1)) a = T(1);
+    double inc = acosd(a);                                                 // :1600
     double R = table_eval(M.zdeg, M.zRpp, M.nz, RTMC_EXTRAP_LINEAR, inc);  // :1679
-    return (uA <= R) ? 0 : 1;                                            // :1681-1682
+    return (uA <= R) ? 0 : 1;                                              // :1681-1682
   }
-  double eSH[3];
+  T eSH[3];
   cross3(d, nv, eSH);  // :1613
-  double nsh = sqrt(sumsq3(eSH));
-  if (nsh < kEps) {  // :1616-1625
-    double t[3] = {1.0, 0.0, 0.0};
-    double tn = dot3(t, nv);
+  T nsh = sqrt(sumsq3(eSH));
+  if (nsh < (T)kEps) {  // :1616-1625
+    T t[3] = {T(1), T(0), T(0)};
+    T tn = dot3(t, nv);
     for (int k = 0; k < 3; ++k) t[k] = t[k] - tn * nv[k];
     for (int k = 0; k < 3; ++k) eSH[k] = t[k];
     nsh = sqrt(sumsq3(t));
   }
   for (int k = 0; k < 3; ++k) eSH[k] = eSH[k] / nsh;
-  double eSV[3];
+  T eSV[3];
   cross3(eSH, d, eSV);  // :1630-1631
-  double nsv = sqrt(sumsq3(eSV));
+  T nsv = sqrt(sumsq3(eSV));
   for (int k = 0; k < 3; ++k) eSV[k] = eSV[k] / nsv;
-  double aSV = dot3(q, eSV), aSH = dot3(q, eSH);  // :1634-1643
-  double ESV = aSV * aSV, ESH = aSH * aSH;
-  double Etot = ESV + ESH + kEps;
-  double probSV = ESV / Etot;
-  if (!(uA < probSV)) return 2;  // :1646, SH always reflects :1720
-  double a = fabs(dot3(d, nv));
-  if (a > 1) a = 1;
+  T aSV = dot3(q, eSV), aSH = dot3(q, eSH);  // :1634-1643
+  T ESV = aSV * aSV, ESH = aSH * aSH;
+  T Etot = ESV + ESH + (T)kEps;
+  T probSV = ESV / Etot;
+  if (!(uA < (double)probSV)) return 2;  // :1646, SH always reflects :1720
+  T a = fabs(dot3(d, nv));
+  if (a > T(1)) a = T(1);
   double inc = acosd(a);
   double R = table_eval(M.zdeg, M.zRss, M.nz, RTMC_EXTRAP_LINEAR, inc);  // :1700
-  return (uB <= R) ? 3 : 4;                                            // :1702-1703
+  return (uB <= R) ? 3 : 4;                                              // :1702-1703
 }
 
 // Applies the reflection of a particle that crossed plane boundary B
 // (acoustic :184-187; elastic :190-321).
-__device__ __forceinline__ void plane_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, double x[3],
-                                              double d[3], double q[3], int& isP, double uA, double uB) {
+template <typename T>
+__device__ __forceinline__ void plane_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, T x[3], T d[3],
+                                              T q[3], int& isP, double uA, double uB) {
   int b = B.dir - 1;
+  const T v2 = (T)(2 * B.val);
   if (acoustic) {
-    x[b] = (2 * B.val) - x[b];
+    x[b] = v2 - x[b];
     d[b] = -d[b];
     return;
   }
   int br = interface_plane(M, b, d, q, isP, uA, uB);
-  x[b] = (2 * B.val) - x[b];  // :198
+  x[b] = v2 - x[b];  // :198
   if (br == 0 || br == 2 || br == 3) {
     d[b] = -d[b];
     q[b] = -q[b];
   } else if (br == 1) {  // P -> SV :212-248
     isP = 0;
-    double ratio = M.vs / M.vp;
-    double dn = d[b];
-    double t[3] = {d[0], d[1], d[2]};
-    t[b] = 0.0;
+    T ratio = (T)(M.vs / M.vp);
+    T dn = d[b];
+    T t[3] = {d[0], d[1], d[2]};
+    t[b] = T(0);
     for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
-    double sin2 = sumsq3(t);
-    double cv = sqrt(fmax(0.0, 1.0 - sin2));
+    T sin2 = sumsq3(t);
+    T cv = sqrt(fmax(T(0), T(1) - sin2));
     t[b] = -sgn(dn) * cv;
-    double nn = sqrt(sumsq3(t));
+    T nn = sqrt(sumsq3(t));
     for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
-    double nv[3] = {0.0, 0.0, 0.0};
-    nv[b] = 1.0;
-    double ax[3];
+    T nv[3] = {T(0), T(0), T(0)};
+    nv[b] = T(1);
+    T ax[3];
     cross3(d, nv, ax);
-    double na = sqrt(sumsq3(ax));
+    T na = sqrt(sumsq3(ax));
     for (int k = 0; k < 3; ++k) ax[k] = ax[k] / na;
     cross3(ax, d, q);
   } else {  // SV -> P :274-320
     isP = 1;
-    double ratio = M.vp / M.vs;
-    double dn = d[b];
-    double t[3] = {d[0], d[1], d[2]};
-    t[b] = 0.0;
+    T ratio = (T)(M.vp / M.vs);
+    T dn = d[b];
+    T t[3] = {d[0], d[1], d[2]};
+    t[b] = T(0);
     for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
-    double sin2 = sumsq3(t);
-    if (sin2 <= 1) {
-      double cv = sqrt(1 - sin2);
+    T sin2 = sumsq3(t);
+    if (sin2 <= T(1)) {
+      T cv = sqrt(T(1) - sin2);
       t[b] = -sgn(dn) * cv;
-      double nn = sqrt(sumsq3(t));
+      T nn = sqrt(sumsq3(t));
       for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
     } else {
       isP = 0;  // total reflection fallback :313-319
@@ -273,93 +288,95 @@ __device__ __forceinline__ void plane_reflect(const MatDev& M, bool acoustic, co
   }
 }
 
-__device__ __forceinline__ void reflect_dq(double d[3], double q[3], const double nv[3]) {
-  double dtn = dot3(d, nv);
-  double ptn = dot3(q, nv);
+template <typename T>
+__device__ __forceinline__ void reflect_dq(T d[3], T q[3], const T nv[3]) {
+  T dtn = dot3(d, nv);
+  T ptn = dot3(q, nv);
   for (int k = 0; k < 3; ++k) {
-    d[k] = d[k] - 2 * dtn * nv[k];
-    q[k] = q[k] - 2 * ptn * nv[k];
+    d[k] = d[k] - T(2) * dtn * nv[k];
+    q[k] = q[k] - T(2) * ptn * nv[k];
   }
 }
 
 // Applies the reflection of a particle outside cylinder B
 // (propagateParticleSmallDt.m:344-578).
-__device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, double x[3],
-                                            double d[3], double q[3], int& isP, double uA, double uB) {
-  double r = sqrt(x[0] * x[0] + x[1] * x[1]);
-  double ov = r - B.val;
-  double nv[3] = {x[0] / r, x[1] / r, 0.0};
-  for (int k = 0; k < 3; ++k) x[k] = x[k] - 2 * ov * nv[k];  // :352
-  if (acoustic) {                                            // :354-359
-    double dp = dot3(d, nv);
-    for (int k = 0; k < 3; ++k) d[k] = d[k] - 2 * dp * nv[k];
+template <typename T>
+__device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, T x[3], T d[3],
+                                            T q[3], int& isP, double uA, double uB) {
+  T r = sqrt(x[0] * x[0] + x[1] * x[1]);
+  T ov = r - (T)B.val;
+  T nv[3] = {x[0] / r, x[1] / r, T(0)};
+  for (int k = 0; k < 3; ++k) x[k] = x[k] - T(2) * ov * nv[k];  // :352
+  if (acoustic) {                                               // :354-359
+    T dp = dot3(d, nv);
+    for (int k = 0; k < 3; ++k) d[k] = d[k] - T(2) * dp * nv[k];
     return;
   }
   if (isP) {  // :373-455
-    double ct = dot3(d, nv);
-    double thd = acosd(fmin(1.0, fmax(-1.0, ct)));
+    T ct = dot3(d, nv);
+    double thd = acosd(fmin(T(1), fmax(T(-1), ct)));
     double R = table_eval(M.zdeg, M.zRpp, M.nz, RTMC_EXTRAP_NEAREST, thd);
     if (uA <= R) {
       reflect_dq(d, q, nv);
     } else {
       isP = 0;
-      double eta = M.vs / M.vp;
-      double ddn = dot3(d, nv);
-      double dt_[3], dnt[3];
+      T eta = (T)(M.vs / M.vp);
+      T ddn = dot3(d, nv);
+      T dt_[3], dnt[3];
       for (int k = 0; k < 3; ++k) {
-        double dnv = ddn * nv[k];
+        T dnv = ddn * nv[k];
         dt_[k] = d[k] - dnv;
       }
       for (int k = 0; k < 3; ++k) dnt[k] = eta * dt_[k];
-      double sin2 = sumsq3(dnt);
-      double cv = sqrt(fmax(0.0, 1 - sin2));
-      double dnew[3];
+      T sin2 = sumsq3(dnt);
+      T cv = sqrt(fmax(T(0), T(1) - sin2));
+      T dnew[3];
       for (int k = 0; k < 3; ++k) dnew[k] = dnt[k] - cv * nv[k];
-      double nn = sqrt(sumsq3(dnew));
+      T nn = sqrt(sumsq3(dnew));
       for (int k = 0; k < 3; ++k) dnew[k] = dnew[k] / nn;
       for (int k = 0; k < 3; ++k) d[k] = dnew[k];
-      double ax[3];
+      T ax[3];
       cross3(dnew, nv, ax);
-      double al = sqrt(sumsq3(ax));
-      if (al < 1e-6) {
-        double e1[3] = {1.0, 0.0, 0.0};
+      T al = sqrt(sumsq3(ax));
+      if (al < T(1e-6)) {
+        T e1[3] = {T(1), T(0), T(0)};
         cross3(dnew, e1, ax);
       }
-      double an = sqrt(sumsq3(ax));
+      T an = sqrt(sumsq3(ax));
       for (int k = 0; k < 3; ++k) ax[k] = ax[k] / an;
       cross3(ax, dnew, q);
     }
   } else {  // :458-576
-    double ai[3];
+    T ai[3];
     cross3(d, nv, ai);
-    double an = sqrt(sumsq3(ai));
+    T an = sqrt(sumsq3(ai));
     for (int k = 0; k < 3; ++k) ai[k] = ai[k] / an;
-    double shc = dot3(q, ai);
-    double prob_sh = shc * shc;
-    if (uA < prob_sh) {
+    T shc = dot3(q, ai);
+    T prob_sh = shc * shc;
+    if (uA < (double)prob_sh) {
       reflect_dq(d, q, nv);
     } else {
-      double ct = dot3(d, nv);
-      double thd = acosd(fmin(1.0, fmax(-1.0, ct)));
+      T ct = dot3(d, nv);
+      double thd = acosd(fmin(T(1), fmax(T(-1), ct)));
       double R = table_eval(M.zdeg, M.zRss, M.nz, RTMC_EXTRAP_NEAREST, thd);
       if (uB <= R) {
         reflect_dq(d, q, nv);
       } else {
         isP = 1;
-        double eta = M.vp / M.vs;
-        double ddn = dot3(d, nv);
-        double dt_[3], dnt[3];
+        T eta = (T)(M.vp / M.vs);
+        T ddn = dot3(d, nv);
+        T dt_[3], dnt[3];
         for (int k = 0; k < 3; ++k) {
-          double dnv = ddn * nv[k];
+          T dnv = ddn * nv[k];
           dt_[k] = d[k] - dnv;
         }
         for (int k = 0; k < 3; ++k) dnt[k] = eta * dt_[k];
-        double sin2 = sumsq3(dnt);
-        if (sin2 <= 1) {
-          double cv = sqrt(1 - sin2);
-          double dnew[3];
+        T sin2 = sumsq3(dnt);
+        if (sin2 <= T(1)) {
+          T cv = sqrt(T(1) - sin2);
+          T dnew[3];
           for (int k = 0; k < 3; ++k) dnew[k] = dnt[k] - cv * nv[k];
-          double nn = sqrt(sumsq3(dnew));
+          T nn = sqrt(sumsq3(dnew));
           for (int k = 0; k < 3; ++k) d[k] = dnew[k] / nn;
         } else {
           isP = 0;
@@ -374,19 +391,19 @@ __device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, cons
 // Scattering of one particle (propagateParticleSmallDt.m:604-639,
 // propagateParticle.m:33-72).  bit2d: the 2-D randi([0 1]) sign bit;
 // uphi: 3-D azimuth uniform; uth: polar-angle uniform; uconv: the P2P (P)
-// or S2S (S) uniform.  Returns theta.
-template <int DIM, typename TabF>
-__device__ __forceinline__ double scatter_angles_t(const MatDev& M, bool acoustic, double d[3], double q[3], int& isP,
+// or S2S (S) uniform.  Returns theta (from the double-precision table).
+template <int DIM, typename T, typename TabF>
+__device__ __forceinline__ double scatter_angles_t(const MatDev& M, bool acoustic, T d[3], T q[3], int& isP,
                                                    bool bit2d, double uphi, double uth, double uconv, TabF lookup) {
   if (DIM == 2) {
-    double s = bit2d ? -1.0 : 1.0;
+    T s = bit2d ? T(-1) : T(1);
     for (int k = 0; k < 3; ++k) q[k] = s * q[k];
   } else {
-    double phi = kTwoPi * uphi;
-    double c[3];
+    T phi = (T)(kTwoPi * uphi);
+    T c[3];
     cross3(d, q, c);
-    double sp, cp;
-    sincos(phi, &sp, &cp);
+    T sp, cp;
+    sc(phi, &sp, &cp);
     for (int k = 0; k < 3; ++k) q[k] = cp * q[k] + sp * c[k];
   }
   if (acoustic) return lookup(0, uth);
@@ -411,15 +428,16 @@ __device__ __forceinline__ double scatter_angles(const MatDev& M, double d[3], d
 
 // dir <- cos(th) dir + sin(th) perp, perp <- -sin(th) dir + cos(th) perp,
 // both renormalised (propagateParticleSmallDt.m:635-639).
-__device__ __forceinline__ void rotate_renorm(double d[3], double q[3], double th) {
-  double st, ct;
-  sincos(th, &st, &ct);
-  double nd[3], np[3];
+template <typename T>
+__device__ __forceinline__ void rotate_renorm(T d[3], T q[3], T th) {
+  T st, ct;
+  sc(th, &st, &ct);
+  T nd[3], np[3];
   for (int k = 0; k < 3; ++k) nd[k] = ct * d[k] + st * q[k];
-  double nrm = sqrt(sumsq3(nd));
+  T nrm = sqrt(sumsq3(nd));
   for (int k = 0; k < 3; ++k) nd[k] = nd[k] / nrm;
   for (int k = 0; k < 3; ++k) np[k] = -st * d[k] + ct * q[k];
-  double nrp = sqrt(sumsq3(np));
+  T nrp = sqrt(sumsq3(np));
   for (int k = 0; k < 3; ++k) np[k] = np[k] / nrp;
   for (int k = 0; k < 3; ++k) {
     d[k] = nd[k];
@@ -427,19 +445,21 @@ __device__ __forceinline__ void rotate_renorm(double d[3], double q[3], double t
   }
 }
 
-__device__ __forceinline__ void renorm(double d[3], double q[3]) {
-  double nrm = sqrt(sumsq3(d));
+template <typename T>
+__device__ __forceinline__ void renorm(T d[3], T q[3]) {
+  T nrm = sqrt(sumsq3(d));
   for (int k = 0; k < 3; ++k) d[k] = d[k] / nrm;
-  double nrp = sqrt(sumsq3(q));
+  T nrp = sqrt(sumsq3(q));
   for (int k = 0; k < 3; ++k) q[k] = q[k] / nrp;
 }
 
 // ------------------------------------------------------------------
 // Observation variables (observeTime.m:15-45): 0 x|r, 1 y|azimuth,
 // 2 z|sin(elevation), 3 direction.  `need` masks the ones to compute.
-__device__ __forceinline__ void obs_vars(int dim, int frame, const double X[3], double dz, unsigned need,
-                                         double v[4]) {
-  double r = 0.0, th = 0.0, ph = 0.0;
+// Computed in the flight precision T, returned in double for the binning.
+template <typename T>
+__device__ __forceinline__ void obs_vars(int dim, int frame, const T X[3], T dz, unsigned need, double v[4]) {
+  T r = T(0), th = T(0), ph = T(0);
   bool polar = frame != RTMC_FRAME_CARTESIAN;
   if (dim == 2) {
     if (polar && (need & 1u)) r = hypot(X[0], X[1]);
@@ -448,16 +468,16 @@ __device__ __forceinline__ void obs_vars(int dim, int frame, const double X[3], 
     if (need & 1u) r = hypot(X[0], X[1]);
     if (need & 2u) th = atan2(X[1], X[0]);
   } else if (frame == RTMC_FRAME_SPHERICAL) {
-    double hxy = 0.0;
+    T hxy = T(0);
     if (need & 5u) hxy = hypot(X[0], X[1]);
     if (need & 1u) r = hypot(hxy, X[2]);
     if (need & 2u) th = atan2(X[1], X[0]);
     if (need & 4u) ph = sin(atan2(X[2], hxy));
   }
-  double cp = dz;
-  if (cp > 1) cp = 1;
-  if (cp < -1) cp = -1;
-  double psi = 0.0;
+  T cp = dz;
+  if (cp > T(1)) cp = T(1);
+  if (cp < T(-1)) cp = T(-1);
+  T psi = T(0);
   if (need & 8u) psi = (dim == 3) ? cp : acos(cp);
   if (frame == RTMC_FRAME_CARTESIAN) {
     v[0] = X[0];

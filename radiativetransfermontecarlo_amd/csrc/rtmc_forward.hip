@@ -57,6 +57,7 @@ struct FwdParams {
   int edges_total;    // doubles of the LDS edge copy (0: edges read from global memory)
   unsigned long long* work;  // global particle counter of the launch (zeroed)
   int chunk;          // particles fetched per counter access
+  int fp32;           // flight arithmetic in FP32 (RTMC_PREC_FP32_FLIGHT)
   int batch;          // lanes waiting for an event before a wavefront takes an event step
   int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
   int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
@@ -85,13 +86,38 @@ struct Rng {
     has = true;
     return u53(r.x, r.y);
   }
+  // uniform in (0, 1] with 32 random bits (FP32 flight): one Philox word each
+  __device__ __forceinline__ double next32() {
+    if (n32 == 0) {
+      U4 in;
+      in.x = c;
+      in.y = 0u;
+      in.z = plo;
+      in.w = phi;
+      w32 = philox10(in, k0, k1);
+      c += 1u;
+      n32 = 4;
+    }
+    uint32_t w = (n32 == 4) ? w32.x : (n32 == 3) ? w32.y : (n32 == 2) ? w32.z : w32.w;
+    n32 -= 1;
+    return ((double)w + 1.0) * (1.0 / 4294967296.0);
+  }
+  U4 w32;
+  int n32;
+  template <typename R>
+  __device__ __forceinline__ double u() {
+    if (sizeof(R) == 8) return next();
+    return next32();
+  }
 };
 
 // index of the next sub-step with a success, trials of probability 1-exp(-lam)
+template <typename R>
 __device__ __forceinline__ int64_t geom_after(Rng& r, double lam) {
-  double u = r.next();
+  double u = r.u<R>();
   if (!(lam > 0.0)) return kInf;
-  double g = ceil(-log(u) / lam);
+  double e = (sizeof(R) == 8) ? -log(u) : (double)(-logf((float)u));
+  double g = ceil(e / lam);
   if (g < 1.0) g = 1.0;
   if (g > 1e15) return kInf;
   return (int64_t)g;
@@ -100,8 +126,11 @@ __device__ __forceinline__ int64_t geom_after(Rng& r, double lam) {
 // First sub-step (relative to the segment base) at which the straight
 // segment x = xb + n h d may trigger boundary B; a prediction that is at most
 // one sub-step early (the sub-step is then executed in full and found clear).
-__device__ __forceinline__ int64_t predict_boundary(const rtmc_boundary& B, const double xb[3], const double d[3],
-                                                    double h) {
+template <typename R>
+__device__ __forceinline__ int64_t predict_boundary(const rtmc_boundary& B, const R xb_[3], const R d_[3], R h_) {
+  const double xb[3] = {xb_[0], xb_[1], xb_[2]};
+  const double d[3] = {d_[0], d_[1], d_[2]};
+  const double h = h_;
   if (B.dir < 4) {
     int b = B.dir - 1;
     double s0 = xb[b] - B.val;
@@ -166,7 +195,23 @@ __device__ __forceinline__ void bin_setup(const FwdParams& p, int dim, const dou
 }
 
 template <int DIM>
-__device__ __forceinline__ void birth_fwd(const FwdParams& p, Rng& rng, double x[3], double d[3], double q[3]) {
+__device__ __forceinline__ void birth_fwd64(const FwdParams& p, Rng& rng, double x[3], double d[3], double q[3]);
+
+template <int DIM, typename R>
+__device__ __forceinline__ void birth_fwd(const FwdParams& p, Rng& rng, R xo[3], R dd[3], R qo[3]) {
+  double x[3], d[3], q[3];
+  birth_fwd64<DIM>(p, rng, x, d, q);
+  for (int k = 0; k < 3; ++k) {
+    xo[k] = (R)x[k];
+    dd[k] = (R)d[k];
+    qo[k] = (R)q[k];
+  }
+}
+
+// birth computed in FP64 for both flight precisions (cheap: once per particle;
+// keeps dir_z = cos(pi/2) of 2-D births at its FP64 value)
+template <int DIM>
+__device__ __forceinline__ void birth_fwd64(const FwdParams& p, Rng& rng, double x[3], double d[3], double q[3]) {
   const rtmc_source& S = p.S;
   if (S.type == RTMC_SRC_POINT) {
     double r;
@@ -243,20 +288,30 @@ __device__ __forceinline__ void birth_fwd(const FwdParams& p, Rng& rng, double x
   }
 }
 
-template <int DIM, bool LDSH>
+template <int DIM, bool LDSH, typename R>
 __device__ __forceinline__ void deposit(const FwdParams& p, const BinSetup& bs, const double* __restrict__ sedges,
-                                        uint32_t* __restrict__ shist, uint32_t* __restrict__ E, const double X[3],
-                                        double dz, int isP, int it, unsigned long long& ndep) {
+                                        uint32_t* __restrict__ shist, uint32_t* __restrict__ E, const R X[3], R dz,
+                                        int isP, int it, unsigned long long& ndep) {
   double v[4];
   obs_vars(DIM, p.G.frame, X, dz, bs.check, v);
   int64_t idx = (int64_t)it * p.time_stride;
+  if (p.edges_total > 0) {
+    // edges in LDS (`sedges` points into the work-group's LDS block)
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (!(bs.check & (1u << k))) continue;
-    const double* e = sedges ? sedges + p.eoff[k] : p.edges[k];
-    int b = bin_guess(e, p.ne[k], bs.e0[k], bs.invw[k], v[k]);
-    if (b < 0) return;
-    idx += (int64_t)b * p.stride[k];
+    for (int k = 0; k < 4; ++k) {
+      if (!(bs.check & (1u << k))) continue;
+      int b = bin_guess(sedges + p.eoff[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
+      if (b < 0) return;
+      idx += (int64_t)b * p.stride[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!(bs.check & (1u << k))) continue;
+      int b = bin_guess(p.edges[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
+      if (b < 0) return;
+      idx += (int64_t)b * p.stride[k];
+    }
   }
   if (!isP) idx += p.mode_stride;
   ndep += 1;
@@ -267,11 +322,11 @@ __device__ __forceinline__ void deposit(const FwdParams& p, const BinSetup& bs, 
   }
 }
 
-template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND>
+template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND, typename R>
 __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __restrict__ E,
                                                  unsigned long long* __restrict__ counters) {
   extern __shared__ double smem[];
-  double* sedges = p.edges_total > 0 ? smem : nullptr;
+  double* sedges = smem;
   __shared__ BinSetup s_bs;
   __shared__ double s_tx0[4], s_tx1[4];
   int* sguide = (int*)(smem + p.edges_total);
@@ -348,9 +403,10 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
   rng.c = 0u;
   rng.has = false;
   rng.spare = 0.0;
-  double xb[3] = {0.0, 0.0, 0.0}, d[3] = {0.0, 0.0, 1.0}, q[3] = {1.0, 0.0, 0.0};
+  rng.n32 = 0;
+  R xb[3] = {R(0), R(0), R(0)}, d[3] = {R(0), R(0), R(1)}, q[3] = {R(1), R(0), R(0)};
   int isP = 1;
-  double h = 0.0;                 // sub-stepped: step length v dt_s; event-driven: speed v
+  R h = R(0);                     // sub-stepped: step length v dt_s; event-driven: speed v
   int64_t kb = 0, Ks = kInf, Kk = kInf, kev = kInf;   // sub-stepped: base, next scatter / removal / event
   double tb = 0.0, ts = INFINITY;                     // event-driven: base time, next scattering time
   int it = 1;                     // next observation index
@@ -409,21 +465,22 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
             rng.phi = (uint32_t)(pid >> 32);
             rng.c = 0u;
             rng.has = false;
+            rng.n32 = 0;
             birth_fwd<DIM>(p, rng, xb, d, q);
             isP = (!AC && p.S.polarization == 1) ? 0 : 1;
             it = 1;
             has = true;
             need = false;
             if (QUANT) {
-              h = (isP ? M.vp : M.vs) * p.dt_sub;
+              h = (R)((isP ? M.vp : M.vs) * p.dt_sub);
               kb = 0;
-              Ks = geom_after(rng, p.lamS[isP ? 0 : 1]);
-              Kk = p.useQ ? geom_after(rng, p.lamK[isP ? 0 : 1]) : kInf;
+              Ks = geom_after<R>(rng, p.lamS[isP ? 0 : 1]);
+              Kk = p.useQ ? geom_after<R>(rng, p.lamK[isP ? 0 : 1]) : kInf;
               next_event();
             } else {
-              h = isP ? M.vp : M.vs;
+              h = (R)(isP ? M.vp : M.vs);
               tb = 0.0;
-              ts = -log(rng.next()) * (isP ? M.mftP : M.mftS);
+              ts = -log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
             }
           } else if (exhausted || r - L >= nb1 - nb0) {
             need = false;  // nothing left to trace
@@ -443,15 +500,15 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
           // sub-step j = kev in full (propagateParticleSmallDt.m:85-642); straight
           // flight from the base to the end of sub-step j-1 first
           const int64_t j = kev;
-          const double m = (double)(j - 1 - kb);
-          double x[3];
+          const R m = (R)(j - 1 - kb);
+          R x[3];
           for (int k = 0; k < 3; ++k) x[k] = xb[k] + (m * h) * d[k];
           bool alive = true;
           if (Kk == j) {  // removed at the start of sub-step j (:100-138)
             alive = false;
             nkill += 1;
           } else {
-            double x1[3] = {x[0], x[1], x[2]};
+            R x1[3] = {x[0], x[1], x[2]};
             for (int k = 0; k < 3; ++k) x[k] = x[k] + h * d[k];
             const int wasP = isP;
             for (int ib = 0; BND && ib < p.G.n_bnd; ++ib) {
@@ -466,8 +523,8 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
               }
               double uA = 0.0, uB = 0.0;
               if (!AC) {
-                uA = rng.next();
-                uB = rng.next();
+                uA = rng.u<R>();
+                uB = rng.u<R>();
               }
               if (B.dir < 4) plane_reflect(M, AC, B, x, d, q, isP, uA, uB);
               else cyl_reflect(M, AC, B, x, d, q, isP, uA, uB);
@@ -476,23 +533,23 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
               if (!AC && isP != wasP) {
                 // mode changed at the boundary: this sub-step's scattering test and
                 // the removal tests from the next one use the new mode
-                h = (isP ? M.vp : M.vs) * p.dt_sub;
-                Ks = j - 1 + geom_after(rng, p.lamS[isP ? 0 : 1]);
-                Kk = p.useQ ? j + geom_after(rng, p.lamK[isP ? 0 : 1]) : kInf;
+                h = (R)((isP ? M.vp : M.vs) * p.dt_sub);
+                Ks = j - 1 + geom_after<R>(rng, p.lamS[isP ? 0 : 1]);
+                Kk = p.useQ ? j + geom_after<R>(rng, p.lamK[isP ? 0 : 1]) : kInf;
               }
               if (Ks == j) {
                 const int before = isP;
-                double uphi = rng.next();
-                double uth = rng.next();
-                double uconv = AC ? 0.0 : rng.next();
+                double uphi = rng.u<R>();
+                double uth = rng.u<R>();
+                double uconv = AC ? 0.0 : rng.u<R>();
                 double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
-                rotate_renorm(d, q, th);
+                rotate_renorm(d, q, (R)th);
                 nscat += 1;
                 if (!AC && isP != before) {
-                  h = (isP ? M.vp : M.vs) * p.dt_sub;
-                  if (p.useQ) Kk = j + geom_after(rng, p.lamK[isP ? 0 : 1]);
+                  h = (R)((isP ? M.vp : M.vs) * p.dt_sub);
+                  if (p.useQ) Kk = j + geom_after<R>(rng, p.lamK[isP ? 0 : 1]);
                 }
-                Ks = j + geom_after(rng, p.lamS[isP ? 0 : 1]);
+                Ks = j + geom_after<R>(rng, p.lamS[isP ? 0 : 1]);
               }
               xb[0] = x[0];
               xb[1] = x[1];
@@ -507,28 +564,28 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
           }
         } else {
           // event-driven: scattering at time ts (propagateParticle.m:24-72)
-          const double f = h * (ts - tb);
+          const R f = (R)((double)h * (ts - tb));
           for (int k = 0; k < 3; ++k) xb[k] = xb[k] + f * d[k];
           tb = ts;
-          double uphi = rng.next();
-          double uth = rng.next();
-          double uconv = AC ? 0.0 : rng.next();
+          double uphi = rng.u<R>();
+          double uth = rng.u<R>();
+          double uconv = AC ? 0.0 : rng.u<R>();
           double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
-          rotate_renorm(d, q, th);
+          rotate_renorm(d, q, (R)th);
           nscat += 1;
-          h = isP ? M.vp : M.vs;
-          ts = tb - log(rng.next()) * (isP ? M.mftP : M.mftS);
+          h = (R)(isP ? M.vp : M.vs);
+          ts = tb - log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
         }
       }
     } else {
       // ---------------- DEPOSIT step ----------------
       if (fly) {
-        double X[3];
+        R X[3];
         if (QUANT) {
-          const double m = (double)((int64_t)it * p.nsub - kb);
+          const R m = (R)((int64_t)it * p.nsub - kb);
           for (int k = 0; k < 3; ++k) X[k] = xb[k] + (m * h) * d[k];
         } else {
-          const double f = h * ((double)it * p.dt_obs - tb);
+          const R f = (R)((double)h * ((double)it * p.dt_obs - tb));
           for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
         }
         deposit<DIM, LDSH>(p, bs, sedges, shist, E, X, d[2], isP, it, ndep);
@@ -568,10 +625,10 @@ __global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __
 // ------------------------------------------------------------------
 // host side
 
-template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND>
+template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND, typename R>
 static int launch_fwd(const FwdParams& p, size_t smem, unsigned grid, unsigned block, uint32_t* E,
                       unsigned long long* counters, hipStream_t st) {
-  auto kern = k_forward<DIM, LDSH, QUANT, AC, BND>;
+  auto kern = k_forward<DIM, LDSH, QUANT, AC, BND, R>;
   if (smem > 64 * 1024) RTMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)smem));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), smem, st, p, E, counters);
@@ -581,19 +638,30 @@ static int launch_fwd(const FwdParams& p, size_t smem, unsigned grid, unsigned b
 
 // compile-time specialisation: dimension, histogram placement, process,
 // acoustic/elastic, with/without boundaries (the event-driven process has none)
-template <int DIM, bool LDSH>
+template <int DIM, bool LDSH, typename R>
 static int dispatch_fwd(const FwdParams& p, bool quant, bool ac, bool bnd, size_t smem, unsigned grid,
                         unsigned block, uint32_t* E, unsigned long long* counters, hipStream_t st) {
   if (quant) {
     if (ac) {
-      if (bnd) return launch_fwd<DIM, LDSH, true, true, true>(p, smem, grid, block, E, counters, st);
-      return launch_fwd<DIM, LDSH, true, true, false>(p, smem, grid, block, E, counters, st);
+      if (bnd) return launch_fwd<DIM, LDSH, true, true, true, R>(p, smem, grid, block, E, counters, st);
+      return launch_fwd<DIM, LDSH, true, true, false, R>(p, smem, grid, block, E, counters, st);
     }
-    if (bnd) return launch_fwd<DIM, LDSH, true, false, true>(p, smem, grid, block, E, counters, st);
-    return launch_fwd<DIM, LDSH, true, false, false>(p, smem, grid, block, E, counters, st);
+    if (bnd) return launch_fwd<DIM, LDSH, true, false, true, R>(p, smem, grid, block, E, counters, st);
+    return launch_fwd<DIM, LDSH, true, false, false, R>(p, smem, grid, block, E, counters, st);
   }
-  if (ac) return launch_fwd<DIM, LDSH, false, true, false>(p, smem, grid, block, E, counters, st);
-  return launch_fwd<DIM, LDSH, false, false, false>(p, smem, grid, block, E, counters, st);
+  if (ac) return launch_fwd<DIM, LDSH, false, true, false, R>(p, smem, grid, block, E, counters, st);
+  return launch_fwd<DIM, LDSH, false, false, false, R>(p, smem, grid, block, E, counters, st);
+}
+
+template <typename R>
+static int dispatch_prec(const FwdParams& p, int dim, bool ldsh, bool quant, bool ac, bool bnd, size_t smem,
+                         unsigned grid, unsigned block, uint32_t* E, unsigned long long* cnt, hipStream_t st) {
+  if (dim == 2) {
+    if (ldsh) return dispatch_fwd<2, true, R>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
+    return dispatch_fwd<2, false, R>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
+  }
+  if (ldsh) return dispatch_fwd<3, true, R>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
+  return dispatch_fwd<3, false, R>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
 }
 
 }  // namespace rtmc
@@ -614,7 +682,8 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
   RTMC_REQUIRE(prob->n_particles >= 0, "n_particles must be >= 0");
   RTMC_REQUIRE(prob->n_times >= 1, "n_times must be >= 1");
   RTMC_REQUIRE(prob->time_steps == 0 || prob->time_steps == 1, "material.timeSteps must be 0 or 1");
-  RTMC_REQUIRE(prob->precision == RTMC_PREC_FP64, "only RTMC_PREC_FP64 is implemented");
+  RTMC_REQUIRE(prob->precision == RTMC_PREC_FP64 || prob->precision == RTMC_PREC_FP32_FLIGHT,
+               "precision must be RTMC_PREC_FP64 or RTMC_PREC_FP32_FLIGHT");
   RTMC_REQUIRE(prob->source.type == RTMC_SRC_POINT || dim == 3, "plane sources are 3-D only");
   if (prob->source.type == RTMC_SRC_POINT && prob->source.radial.n != 0) {
     e = check_table(&prob->source.radial, "source.radial");
@@ -723,6 +792,7 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
 
   // dynamic work distribution: one zeroed counter per launch
   p.chunk = 256;
+  p.fp32 = prob->precision == RTMC_PREC_FP32_FLIGHT;
   p.batch = 24;
   if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
   RTMC_REQUIRE(p.batch >= 1 && p.batch <= 64, "RTMC_EVENT_BATCH must be in [1, 64]");
@@ -742,10 +812,6 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
 
 static int launch_all(const FwdParams& p, int dim, bool ldsh, bool quant, bool ac, bool bnd, size_t smem,
                       unsigned grid, unsigned block, uint32_t* E, unsigned long long* cnt, hipStream_t st) {
-  if (dim == 2) {
-    if (ldsh) return dispatch_fwd<2, true>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
-    return dispatch_fwd<2, false>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
-  }
-  if (ldsh) return dispatch_fwd<3, true>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
-  return dispatch_fwd<3, false>(p, quant, ac, bnd, smem, grid, block, E, cnt, st);
+  if (p.fp32) return dispatch_prec<float>(p, dim, ldsh, quant, ac, bnd, smem, grid, block, E, cnt, st);
+  return dispatch_prec<double>(p, dim, ldsh, quant, ac, bnd, smem, grid, block, E, cnt, st);
 }

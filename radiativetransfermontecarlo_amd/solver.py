@@ -400,7 +400,8 @@ def _default_seed() -> int:
 def forward_histogram(geometry, source, material: MaterialClass, O: Observation, N: int, *, seed: int,
                       device, time_steps: int = 0, particle_offset: int = 0, E: Optional[torch.Tensor] = None,
                       counters: Optional[torch.Tensor] = None, block_size: int = 0, hist_mode: int = 0,
-                      particles_per_thread: int = 0, dmat: Optional[DeviceMaterial] = None):
+                      particles_per_thread: int = 0, dmat: Optional[DeviceMaterial] = None,
+                      precision: str = "fp64"):
     """One rtmc_forward call: adds the counts of N particles into E (flat int32
     device tensor, allocated when None).  Returns (E, counters, keep-alive)."""
     d = int(_get(geometry, 'dimension'))
@@ -420,7 +421,9 @@ def forward_histogram(geometry, source, material: MaterialClass, O: Observation,
     prob.n_times = int(O.Nt)
     prob.time_steps = int(time_steps)
     prob.dt = float(np.mean(np.diff(O.t))) if O.Nt > 1 else 0.0
-    prob.precision = _abi.PREC_FP64
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("precision must be 'fp64' or 'fp32' (FP32 free flight, FP64 time)")
+    prob.precision = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_FP32_FLIGHT
     prob.block_size = int(block_size)
     prob.particles_per_thread = int(particles_per_thread)
     prob.hist_mode = int(hist_mode)
@@ -435,12 +438,14 @@ def forward_histogram(geometry, source, material: MaterialClass, O: Observation,
 
 
 def radiativeTransfer(geometry, source, material: MaterialClass, observation, *, seed: Optional[int] = None,
-                      engine: str = "fused", device=None, allow_nd: bool = False, return_histogram: bool = False):
+                      engine: str = "fused", device=None, allow_nd: bool = False, return_histogram: bool = False,
+                      precision: str = "fp64"):
     """obs = radiativeTransfer(geometry, source, material, observation)
     (radiativeTransfer.m:1-207) on one GPU.
 
     engine='fused': one persistent kernel for all packets (Philox draws).
     engine='functions': the reference's packet x time loop of function calls.
+    precision='fp32' (fused engine): FP32 direction/flight arithmetic, FP64 time.
     """
     d = int(_get(geometry, 'dimension'))
     if isinstance(geometry, dict) and 'frame' not in geometry:
@@ -467,7 +472,7 @@ def radiativeTransfer(geometry, source, material: MaterialClass, observation, *,
     if engine == "fused":
         dmat = DeviceMaterial(material, d, dev, with_zoeppritz=bool(_get(geometry, 'bnd', None)))
         E, counters, _k = forward_histogram(geometry, source, material, O, N, seed=seed, device=dev,
-                                            time_steps=time_steps, dmat=dmat)
+                                            time_steps=time_steps, dmat=dmat, precision=precision)
     elif engine == "functions":
         E = _functions_engine(geometry, source, material, O, N, Np, seed, dev, time_steps)
     else:

@@ -169,3 +169,41 @@ def test_paasschens_2d_exact_solution(gpu):
         sel = (r > 0.5) & (r < 0.7 * 2.0 * t)
         rel = dens[sel, 0, it] / ref[sel] - 1
         assert np.max(np.abs(rel)) < 0.03, (t, np.max(np.abs(rel)))
+
+
+@pytest.mark.parametrize("case", ["config1", "elastic2d", "halfspace"])
+def test_fp32_flight_statistics(gpu, case):
+    """RTMC_PREC_FP32_FLIGHT: FP32 direction/position arithmetic, FP64 time.
+    Per-bin agreement with the FP64 oracle within Monte Carlo error (the FP32
+    rounding of positions, ~1e-7 relative, is far below the bin widths)."""
+    if case == "config1":
+        geometry, source, material, observation = cases.config1()
+        observation = dict(observation, time=np.arange(0, 2 + 1e-9, 0.02))
+        n_cpu = 100_000
+    elif case == "elastic2d":
+        geometry = {'dimension': 2, 'frame': 'cartesian'}
+        source = {'lambda': 0.01, 'position': [0, 0]}
+        e = np.linspace(-3, 3, 25)
+        observation = {'x': e, 'y': e, 'time': np.arange(0, 0.8 + 1e-9, 0.05)}
+        material = cases.elastic_iso(2)
+        material.Q = [40.0, 20.0]
+        material.Frequency = 1.0
+        n_cpu = 60_000
+    else:
+        geometry = {'dimension': 3, 'frame': 'cylindrical', 'bnd': [{'dir': 3, 'val': 0.0}]}
+        source = {'lambda': 0.05, 'position': [0, 0, -0.3]}
+        observation = {'x': np.linspace(0, 4, 21), 'z': np.linspace(-4, 0, 9),
+                       'time': np.arange(0, 0.6 + 1e-9, 0.05)}
+        material = cases.elastic_iso(3, rho=2000)
+        n_cpu = 40_000
+    d = geometry['dimension']
+    material = MaterialClass.prepareSigma(material, d)
+    O = initializeObservation(geometry, bool(material.acoustics), observation, 1)
+    E32, cnt, _ = solver.forward_histogram(geometry, source, material, O, 1_000_000, seed=21, device='cuda',
+                                           precision='fp32')
+    Eg = E32.cpu().numpy().view(np.uint32)
+    if case == "config1":
+        per_t = Eg.reshape(O.shape, order='F').sum(axis=(0, 1, 3))
+        assert np.all(per_t[1:] == 1_000_000)
+    Eo, _ = orc.forward(geometry, source, material, O, n_cpu, seed=22, packet=10_000)
+    assert_same_distribution(Eg, 1_000_000, Eo, n_cpu, f"fp32 flight {case}")
