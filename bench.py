@@ -1,6 +1,6 @@
 """Benchmark: scattering events/s of the fused forward kernel on one MI355X.
 
-Workload: 2-D elastic P-SV medium (vp = 6, vs = 3.46, f = 5, cv = 0.1, Gaussian
+Default workload (config 2): 2-D elastic P-SV medium (vp = 6, vs = 3.46, f = 5, cv = 0.1, Gaussian
 ACF, lc = 0.5, Q = 500), cartesian frame, P point source, 1e7 particles,
 x, y = -50:0.5:50, 36 direction bins, t = 0:0.05:20 -- a 4.6 GB uint32
 histogram (x, y, direction, time, mode) resident in HBM.  One step = one full
@@ -12,6 +12,10 @@ Prints one JSON line (see the repository's benchmark contract) with a
 histogram deposit, a 4-byte read and write of the counter) and a
 `cpu_baseline` measured with the CPU oracle on a bounded sample of the same
 workload on this host's cores.
+
+`--config 1` times the reference's own README case instead (3-D acoustic,
+1e6 particles, 120 x 251 histogram held in LDS); its roofline is FP64 VALU
+(175 flop-equivalents per 3-D scattering event, 24 per radial deposit).
 """
 import argparse
 import json
@@ -24,7 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_DEPOSIT = 8
+FLOP_PER_EVENT_3D = 175
+FLOP_PER_DEPOSIT_RADIAL = 24
 
 
 def parse():
@@ -32,7 +39,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--config", type=int, choices=[1, 2], default=2)
+    ap.add_argument("--particles", type=int, default=0, help="0 = the configuration's own count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
@@ -67,7 +75,7 @@ def cpu_baseline(args, geometry, source, material, O):
                                  return_counters=True)
     t = time.perf_counter() - t0
     return {"value": nscat / t, "unit": "scattering events/s", "cores": threads, "kind": "port",
-            "sample": f"{n} particles of the same workload (full 20 s, all 401 times), sub-step loop of "
+            "sample": f"{n} particles of the same workload (all {O.Nt} observation times), sub-step loop of "
                       f"propagateParticleSmallDt restated in C (FP64, OpenMP over packets of 2000), {t:.1f} s; "
                       f"{dep / t:.3e} deposits/s"}
 
@@ -86,11 +94,21 @@ def main():
     from tests import cases
 
     dev = torch.device("cuda", 0)
-    geometry, source, material, observation = cases.config2(n=args.particles)
-    N = args.particles
-    material = MaterialClass.prepareSigma(material, 2)
-    O = initializeObservation(geometry, False, observation, N, allow_nd=True)
-    dmat = solver.DeviceMaterial(material, 2, dev)
+    if args.config == 2:
+        N = args.particles or 10_000_000
+        geometry, source, material, observation = cases.config2(n=N)
+        workload = ("2-D elastic P-SV, cartesian, 1e7 particles, Q=500, x,y=-50:0.5:50, 36 direction bins, "
+                    "t=0:0.05:20 (config 2)")
+    else:
+        N = args.particles or 1_000_000
+        geometry, source, material, observation = cases.config1(n=N)
+        workload = ("3-D acoustic, spherical, 1e6 particles, exp ACF lc=0.1, f=10, x=0:0.05:6, t=0:0.02:5 "
+                    "(config 1, the reference's README case)")
+    args.particles = N
+    d = geometry['dimension']
+    material = MaterialClass.prepareSigma(material, d)
+    O = initializeObservation(geometry, bool(material.acoustics), observation, N, allow_nd=True)
+    dmat = solver.DeviceMaterial(material, d, dev)
     E = torch.zeros(O.size, dtype=torch.int32, device=dev)
     cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -125,7 +143,17 @@ def main():
     ndep = int(c[_abi.CNT_DEPOSIT])
     ms = 1000.0 * wall / args.steps
     kavg = float(np.mean(kern_ms))
-    achieved = BYTES_PER_DEPOSIT * ndep / (kavg * 1e-3) / 1e9
+    if args.config == 2:
+        achieved = BYTES_PER_DEPOSIT * ndep / (kavg * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                "kernel": "k_forward", "kernel_ms": kavg,
+                "algorithmic_unit": "8 B per histogram deposit (4 B read + 4 B write of the counter)"}
+    else:
+        tf = (FLOP_PER_EVENT_3D * nscat + FLOP_PER_DEPOSIT_RADIAL * ndep) / (kavg * 1e-3) / 1e12
+        roof = {"bound": "fp64-valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_forward", "kernel_ms": kavg,
+                "algorithmic_unit": "175 flop-eq per 3-D scattering event + 24 per radial deposit"}
     out = {
         "metric": "scattering events/s on 1 MI355X",
         "value": nscat / (ms * 1e-3),
@@ -139,13 +167,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 flight / f64 time",
         "data": "synthetic (Philox4x32-10 particles; material tables from the reference's formulas)",
-        "config": {"workload": "2-D elastic P-SV, cartesian, 1e7 particles, Q=500, x,y=-50:0.5:50, 36 direction "
-                               "bins, t=0:0.05:20 (config 2)", "particles": N, "histogram_bytes": int(O.size) * 4,
+        "config": {"workload": workload, "particles": N, "histogram_bytes": int(O.size) * 4,
                    "deposits_per_step": ndep, "scatter_events_per_step": nscat,
                    "deposits_per_s": ndep / (ms * 1e-3)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_forward", "kernel_ms": kavg},
+        "roofline": roof,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, geometry, source, material, O)

@@ -322,8 +322,10 @@ __device__ __forceinline__ void deposit(const FwdParams& p, const BinSetup& bs, 
   }
 }
 
+// LDS-histogram kernels run one 1024-lane work-group per CU (the histogram
+// takes most of the LDS); the others use 256-lane work-groups.
 template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND, typename R>
-__global__ __launch_bounds__(256) void k_forward(const FwdParams p, uint32_t* __restrict__ E,
+__global__ __launch_bounds__(LDSH ? 1024 : 256) void k_forward(const FwdParams p, uint32_t* __restrict__ E,
                                                  unsigned long long* __restrict__ counters) {
   extern __shared__ double smem[];
   double* sedges = smem;
@@ -778,8 +780,9 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
   p.hist_size = ldsh ? hsize : 0;
   size_t smem = edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0);
 
-  unsigned block = prob->block_size > 0 ? (unsigned)prob->block_size : 256u;
-  RTMC_REQUIRE(block % 64 == 0 && block <= 256, "block_size must be a multiple of 64 and <= 256");
+  unsigned block = prob->block_size > 0 ? (unsigned)prob->block_size : (ldsh ? 1024u : 256u);
+  RTMC_REQUIRE(block % 64 == 0 && block <= (ldsh ? 1024u : 256u),
+               "block_size must be a multiple of 64 and <= 256 (<= 1024 with the LDS histogram)");
   int64_t want = (prob->n_particles + block - 1) / block;
   int64_t per_cu;
   if (ldsh) per_cu = std::max<int64_t>(1, (int64_t)max_smem / (int64_t)std::max<size_t>(smem, 1)) ;
