@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--ppt", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="flight arithmetic of the fused kernel (fp32: FP32 flight, FP64 time)")
     return ap.parse_args()
@@ -52,7 +54,7 @@ def parse():
 
 def traffic_from_profiles():
     """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
-    p = os.path.join(ROOT, "profiles", "k_forward_config2_traffic.json")
+    p = os.path.join(ROOT, "profiles", "k_forward_elastic2d_1e7_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
@@ -88,6 +90,14 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4}))
+        return
     from radiativetransfermontecarlo_amd import _abi, solver
     from radiativetransfermontecarlo_amd.material import MaterialClass
     from radiativetransfermontecarlo_amd.observation import initializeObservation
