@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RTMC_ABI_VERSION 1
+#define RTMC_ABI_VERSION 2
 
 /* status codes */
 #define RTMC_OK 0
@@ -279,6 +279,87 @@ int rtmc_observe(const rtmc_geometry* geometry, const rtmc_bins* bins,
  * propagateParticle.  counters: NULL or device uint64[RTMC_NCOUNTERS]. */
 int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* counters,
                  void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched launches: many materials and sources with their own particle
+ * counts in one persistent kernel.  The reference solves one material per
+ * radiativeTransfer call (radiativeTransfer.m:1-207); a batch replaces a loop
+ * of such calls over materials that share the geometry, the observation bins
+ * and the time axis. */
+
+/* One material + source of a batch. */
+typedef struct {
+    rtmc_material material;
+    rtmc_source source;
+    int64_t n_particles;
+    uint64_t particle_offset;   /* global id of the first particle */
+    uint64_t seed;              /* Philox key; same seed + ids = same trajectories */
+    uint32_t* E;                /* device histogram (rtmc_bins layout); counts are added */
+    uint64_t* counters;         /* device uint64[RTMC_NCOUNTERS] (added to) or NULL */
+} rtmc_batch_item;
+
+/* What the items of a batch share, and the launch configuration. */
+typedef struct {
+    rtmc_geometry geometry;
+    rtmc_bins bins;
+    int32_t n_times;            /* Nt, including t = 0 */
+    int32_t time_steps;         /* 0 sub-stepped, 1 event-driven */
+    double dt;                  /* mean(diff(t)) */
+    int32_t precision;          /* RTMC_PREC_* */
+    int32_t block_size;         /* 0 = library default */
+    int32_t particles_per_thread; /* 0 = library default (fill the device) */
+    int32_t hist_mode;          /* 0 = auto, 1 = global atomics, 2 = LDS-privatised */
+    int32_t event_batch;        /* 0 = default (24): lanes waiting before an event step */
+    int32_t n_items;
+    const rtmc_batch_item* items;   /* host array of n_items */
+} rtmc_batch;
+
+/* rtmc_forward for every item of the batch in one launch.  Item i gives the
+ * same histogram as rtmc_forward with item i's material, source, particles
+ * and seed (the trajectories are the same; only the launch differs). */
+int rtmc_forward_batched(const rtmc_batch* batch, void* stream);
+
+/* Adjoint data of one item.  The coefficients of variation of the material
+ * are c * (cv / cv[0]) with c = cv[0]; lc = CorrelationLength; Q = QP. */
+typedef struct {
+    const float* weight;        /* device: dL/dE per histogram bin (E's layout, raw counts) */
+    double* grad;               /* device double[3]: += dL/dc, dL/dlc, dL/dQ */
+    double* dhist;              /* device double[3 * hist] or NULL: += dE/dc, dE/dlc, dE/dQ
+                                   per bin (hist = bins.time_stride * n_times) */
+    double cv;                  /* c */
+    double dlogSigma_dlc;       /* d log(Sigma) / d lc */
+    const double* dlogsigma_dlc; /* device: d log sigma(theta) / d lc at theta_k = k pi / (n - 1) */
+    int64_t n_dlogsigma;        /* >= 2 */
+} rtmc_adjoint_item;
+
+/* Custom backward of rtmc_forward_batched (no counterpart in the reference,
+ * which has no derivatives): replays every trajectory of the batch from its
+ * Philox counter and accumulates, per item, the likelihood-ratio estimate of
+ * dL/d(c, lc, Q) = sum over deposits of weight[bin] * score(trajectory up to
+ * the deposit), plus optionally the per-bin derivatives dE/d(c, lc, Q).
+ * Acoustic media only.  `batch` must be the forward's batch (same seeds,
+ * particles, materials); items' E are not touched, counters are added to. */
+int rtmc_adjoint_batched(const rtmc_batch* batch, const rtmc_adjoint_item* adjoint,
+                         void* stream);
+
+/* Launch configuration chosen by rtmc_autotune_batched. */
+typedef struct {
+    int32_t block_size;
+    int32_t particles_per_thread;
+    int32_t hist_mode;
+    int32_t event_batch;
+    double ms;                  /* measured time of the chosen configuration on the sample */
+    double ms_default;          /* measured time of the library default on the sample */
+} rtmc_tuning;
+
+/* Times candidate launch configurations (block size, particles per thread,
+ * histogram privatisation, event batch) on a sample of every item
+ * (sample_fraction of its particles, at least 4096) into scratch histograms,
+ * and returns the fastest.  Synchronises `stream`; the items' E and counters
+ * are not touched.  Histograms above 256 MiB in total are not tuned
+ * (RTMC_ERR_ARG). */
+int rtmc_autotune_batched(const rtmc_batch* batch, double sample_fraction,
+                          rtmc_tuning* best, void* stream);
 
 #ifdef __cplusplus
 }

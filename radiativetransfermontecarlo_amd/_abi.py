@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 ERR_ARG = -1
@@ -174,6 +174,58 @@ class Problem(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("material", Material),
+        ("source", Source),
+        ("n_particles", C.c_int64),
+        ("particle_offset", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("E", C.c_void_p),
+        ("counters", C.c_void_p),
+    ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("geometry", Geometry),
+        ("bins", Bins),
+        ("n_times", C.c_int32),
+        ("time_steps", C.c_int32),
+        ("dt", C.c_double),
+        ("precision", C.c_int32),
+        ("block_size", C.c_int32),
+        ("particles_per_thread", C.c_int32),
+        ("hist_mode", C.c_int32),
+        ("event_batch", C.c_int32),
+        ("n_items", C.c_int32),
+        ("items", C.POINTER(BatchItem)),
+    ]
+
+
+class AdjointItem(C.Structure):
+    _fields_ = [
+        ("weight", C.c_void_p),
+        ("grad", C.c_void_p),
+        ("dhist", C.c_void_p),
+        ("cv", C.c_double),
+        ("dlogSigma_dlc", C.c_double),
+        ("dlogsigma_dlc", C.c_void_p),
+        ("n_dlogsigma", C.c_int64),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("block_size", C.c_int32),
+        ("particles_per_thread", C.c_int32),
+        ("hist_mode", C.c_int32),
+        ("event_batch", C.c_int32),
+        ("ms", C.c_double),
+        ("ms_default", C.c_double),
+    ]
+
+
 class SubstepInfo(C.Structure):
     _fields_ = [
         ("T_eff", C.c_double),
@@ -196,6 +248,9 @@ EXPORTS = (
     "rtmc_propagate_events",
     "rtmc_observe",
     "rtmc_forward",
+    "rtmc_forward_batched",
+    "rtmc_adjoint_batched",
+    "rtmc_autotune_batched",
 )
 
 _lib = None
@@ -223,6 +278,12 @@ def _declare(lib):
                                  C.c_int64, C.c_void_p]
     lib.rtmc_forward.restype = C.c_int
     lib.rtmc_forward.argtypes = [P(Problem), C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.rtmc_forward_batched.restype = C.c_int
+    lib.rtmc_forward_batched.argtypes = [P(Batch), C.c_void_p]
+    lib.rtmc_adjoint_batched.restype = C.c_int
+    lib.rtmc_adjoint_batched.argtypes = [P(Batch), P(AdjointItem), C.c_void_p]
+    lib.rtmc_autotune_batched.restype = C.c_int
+    lib.rtmc_autotune_batched.argtypes = [P(Batch), C.c_double, P(Tuning), C.c_void_p]
     return lib
 
 

@@ -1,0 +1,247 @@
+"""Batched forward, adjoint and autotuner (rtmc_forward_batched,
+rtmc_adjoint_batched, rtmc_autotune_batched).
+
+The reference solves one material per ``radiativeTransfer`` call
+(radiativeTransfer.m:1-207) and has no derivatives.  A :class:`Batch` runs many
+materials and sources that share the geometry, the observation bins and the
+time axis in one persistent kernel; :meth:`Batch.adjoint` replays the same
+trajectories and returns dL/d(c, lc, Q) per item, where the coefficients of
+variation of item i are ``c * cv / cv[0]`` (``c = cv[0]``), ``lc`` is
+``CorrelationLength`` and ``Q`` the intrinsic attenuation (acoustic media).
+:class:`EnergyDensity` wraps both as a ``torch.autograd.Function``.
+"""
+from __future__ import annotations
+
+import copy
+import ctypes as C
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _abi
+from .material import MaterialClass, _integral
+from .observation import Observation, normalize
+from .solver import (DeviceBins, DeviceMaterial, _device, _f64, _get, _stream_ptr, build_geometry, build_source)
+
+__all__ = ["prepare_material", "lc_derivatives", "Batch", "EnergyDensity", "count_scale"]
+
+
+def prepare_material(base: MaterialClass, d: int, c: Optional[float] = None, lc: Optional[float] = None,
+                     Q: Optional[float] = None, knot_threshold: float = 1e-8) -> MaterialClass:
+    """A prepared copy of ``base`` with cv scaled to cv[0] = c, lc and Q replaced
+    (MaterialClass.prepareSigma, MaterialClass.m:2379-2442).  The default
+    tables are the reference's; for gradients use ``knot_threshold=0``
+    (tables that follow sigma over the whole angle range)."""
+    m = copy.deepcopy(base)
+    if c is not None:
+        cv = np.asarray(m.coefficients_of_variation, dtype=np.float64).ravel()
+        m.coefficients_of_variation = cv * (float(c) / cv[0])
+    if lc is not None:
+        m.CorrelationLength = float(lc)
+    if Q is not None:
+        m.Q = float(Q)
+    m.Phi = None
+    m.sigma = []
+    m.invcdf = None
+    return MaterialClass.prepareSigma(m, d, knot_threshold=knot_threshold)
+
+
+def _sigma_fn(base: MaterialClass, lc: float):
+    m = copy.deepcopy(base)
+    m.CorrelationLength = float(lc)
+    m.Phi = None
+    m.sigma = []
+    m.CalcSigma()
+    return m.sigma[0]
+
+
+def _Sigma(sigma, d: int) -> float:
+    # total cross-section as in MaterialClass.prepareSigmaOne (MaterialClass.m:2443-2455)
+    if d == 2:
+        return 2 * _integral(sigma, 0, np.pi)
+    return 2 * np.pi * _integral(lambda th: sigma(th) * np.sin(th), 0, np.pi)
+
+
+def lc_derivatives(material: MaterialClass, d: int, n: int = 2049, rel: float = 1e-4):
+    """d log sigma(theta)/d lc on theta = linspace(0, pi, n), and d log Sigma/d lc,
+    by central differences of the acoustic cross-section (MaterialClass.m:204-235)
+    at lc (1 +- rel).  Returns (g, dlogSigma)."""
+    if not material.acoustics:
+        raise ValueError("lc derivatives are implemented for acoustic media")
+    lc = float(material.CorrelationLength)
+    h = rel * lc
+    sp = _sigma_fn(material, lc + h)
+    sm = _sigma_fn(material, lc - h)
+    th = np.linspace(0.0, np.pi, n)
+    a = np.asarray(sp(th), dtype=np.float64)
+    b = np.asarray(sm(th), dtype=np.float64)
+    ok = (a > 0) & (b > 0)
+    g = np.zeros(n)
+    g[ok] = (np.log(a[ok]) - np.log(b[ok])) / (2 * h)
+    Sp, Sm = _Sigma(sp, d), _Sigma(sm, d)
+    dlogS = (math.log(Sp) - math.log(Sm)) / (2 * h)
+    return g, dlogS
+
+
+def count_scale(O: Observation, N: int) -> np.ndarray:
+    """Per-bin factor f with energyDensity = count * f, flat in the histogram's
+    column-major order (radiativeTransfer.m:160-188)."""
+    ones = np.ones(O.size, dtype=np.float64)
+    dens, _ = normalize(O, ones, N)
+    return np.asarray(dens, dtype=np.float64).reshape(-1, order='F')
+
+
+class Batch:
+    """Items (material, source, particle count, seed) sharing geometry and
+    observation; owns their device histograms E (n_items x O.size, int32) and
+    counters (n_items x 4, int64)."""
+
+    def __init__(self, geometry, O: Observation, materials: Sequence[MaterialClass], sources, Ns: Sequence[int],
+                 seeds: Sequence[int], *, device=None, time_steps: int = 0, precision: str = "fp64",
+                 offsets: Optional[Sequence[int]] = None):
+        n = len(materials)
+        if not (len(Ns) == n and len(seeds) == n):
+            raise ValueError("materials, Ns and seeds must have the same length")
+        if not isinstance(sources, (list, tuple)):
+            sources = [sources] * n
+        self.dev = _device(device)
+        self.d = int(_get(geometry, 'dimension'))
+        self.O = O
+        self.n = n
+        self.materials = list(materials)
+        self.Ns = [int(v) for v in Ns]
+        self.keep = []
+        self.geometry = build_geometry(geometry)
+        self.bins = DeviceBins(O, self.dev)
+        self.dmats = [DeviceMaterial(m, self.d, self.dev, with_zoeppritz=self.geometry.n_bnd > 0)
+                      for m in materials]
+        self.E = torch.zeros((n, O.size), dtype=torch.int32, device=self.dev)
+        self.counters = torch.zeros((n, _abi.NCOUNTERS), dtype=torch.int64, device=self.dev)
+        self.items = (_abi.BatchItem * max(n, 1))()
+        for i in range(n):
+            it = self.items[i]
+            it.material = self.dmats[i].c
+            it.source = build_source(sources[i], self.d, self.keep, self.dev)
+            it.n_particles = self.Ns[i]
+            it.particle_offset = int(offsets[i]) if offsets is not None else 0
+            it.seed = int(seeds[i]) & 0xFFFFFFFFFFFFFFFF
+            it.E = self.E[i].data_ptr()
+            it.counters = self.counters[i].data_ptr()
+        b = _abi.Batch()
+        b.geometry = self.geometry
+        b.bins = self.bins.c
+        b.n_times = int(O.Nt)
+        b.time_steps = int(time_steps)
+        b.dt = float(np.mean(np.diff(O.t))) if O.Nt > 1 else 0.0
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32' (FP32 free flight, FP64 time)")
+        b.precision = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_FP32_FLIGHT
+        b.n_items = n
+        b.items = C.cast(self.items, C.POINTER(_abi.BatchItem))
+        self.c = b
+
+    def configure(self, block_size: int = 0, particles_per_thread: int = 0, hist_mode: int = 0,
+                  event_batch: int = 0) -> None:
+        self.c.block_size = int(block_size)
+        self.c.particles_per_thread = int(particles_per_thread)
+        self.c.hist_mode = int(hist_mode)
+        self.c.event_batch = int(event_batch)
+
+    def forward(self, zero: bool = True):
+        """Runs every item; returns (E, counters) (device tensors)."""
+        if zero:
+            self.E.zero_()
+            self.counters.zero_()
+        lib = _abi.load()
+        _abi.check(lib.rtmc_forward_batched(C.byref(self.c), C.c_void_p(_stream_ptr(self.dev))),
+                   "rtmc_forward_batched")
+        return self.E, self.counters
+
+    def autotune(self, sample_fraction: float = 0.02, apply: bool = True) -> dict:
+        """rtmc_autotune_batched on a sample of every item; applies the winner."""
+        t = _abi.Tuning()
+        lib = _abi.load()
+        _abi.check(lib.rtmc_autotune_batched(C.byref(self.c), float(sample_fraction), C.byref(t),
+                                             C.c_void_p(_stream_ptr(self.dev))), "rtmc_autotune_batched")
+        out = {"block_size": t.block_size, "particles_per_thread": t.particles_per_thread,
+               "hist_mode": t.hist_mode, "event_batch": t.event_batch, "ms": t.ms, "ms_default": t.ms_default}
+        if apply:
+            self.configure(t.block_size, t.particles_per_thread, t.hist_mode, t.event_batch)
+        return out
+
+    def derivative_tables(self, n: int = 2049):
+        """(c, dlogSigma_dlc, dlogsigma_dlc table) of every item."""
+        out = []
+        for m in self.materials:
+            g, dS = lc_derivatives(m, self.d, n)
+            c = float(np.asarray(m.coefficients_of_variation, dtype=np.float64).ravel()[0])
+            out.append((c, dS, g))
+        return out
+
+    def adjoint(self, weights: torch.Tensor, tables=None, dhist: bool = False):
+        """dL/d(c, lc, Q) of every item (n x 3 float64 device tensor) for the
+        per-bin weights dL/dcount (n x O.size); with ``dhist`` also the per-bin
+        derivatives d count/d(c, lc, Q) (n x 3 x O.size)."""
+        if tables is None:
+            tables = self.derivative_tables()
+        w = weights.to(device=self.dev, dtype=torch.float32).contiguous()
+        if w.shape != (self.n, self.O.size):
+            raise ValueError(f"weights must have shape {(self.n, self.O.size)}")
+        grad = torch.zeros((self.n, 3), dtype=torch.float64, device=self.dev)
+        dh = torch.zeros((self.n, 3, self.O.size), dtype=torch.float64, device=self.dev) if dhist else None
+        adj = (_abi.AdjointItem * max(self.n, 1))()
+        keep = []
+        for i, (c, dS, g) in enumerate(tables):
+            gt = _f64(g, self.dev)
+            keep.append(gt)
+            a = adj[i]
+            a.weight = w[i].data_ptr()
+            a.grad = grad[i].data_ptr()
+            a.dhist = dh[i].data_ptr() if dh is not None else None
+            a.cv = float(c)
+            a.dlogSigma_dlc = float(dS)
+            a.dlogsigma_dlc = gt.data_ptr()
+            a.n_dlogsigma = int(g.size)
+        lib = _abi.load()
+        _abi.check(lib.rtmc_adjoint_batched(C.byref(self.c), adj, C.c_void_p(_stream_ptr(self.dev))),
+                   "rtmc_adjoint_batched")
+        torch.cuda.current_stream(self.dev).synchronize()  # keeps `keep` alive until the kernel is done
+        return (grad, dh) if dhist else grad
+
+    def energy_density(self) -> torch.Tensor:
+        """energyDensity of every item, flat (n x O.size, float64)."""
+        s = torch.stack([torch.as_tensor(count_scale(self.O, N), device=self.dev) for N in self.Ns])
+        return self.E.to(torch.float64) * s
+
+
+class EnergyDensity(torch.autograd.Function):
+    """energyDensity (n x O.size) of n acoustic materials as a function of
+    their (c, lc, Q) (an n x 3 float64 tensor), with the adjoint kernel as
+    backward.  ``spec`` = dict(base=MaterialClass, geometry, O, sources, Ns,
+    seeds, device, time_steps, precision)."""
+
+    @staticmethod
+    def forward(ctx, params: torch.Tensor, spec: dict):
+        d = int(_get(spec["geometry"], 'dimension'))
+        P = params.detach().cpu().numpy().astype(np.float64)
+        kt = spec.get("knot_threshold", 0.0)
+        mats = [prepare_material(spec["base"], d, c=p[0], lc=p[1], Q=p[2], knot_threshold=kt) for p in P]
+        B = Batch(spec["geometry"], spec["O"], mats, spec["sources"], spec["Ns"], spec["seeds"],
+                  device=spec.get("device"), time_steps=spec.get("time_steps", 0),
+                  precision=spec.get("precision", "fp64"))
+        if spec.get("tuning"):
+            B.configure(**spec["tuning"])
+        B.forward()
+        dens = B.energy_density()
+        ctx.B = B
+        return dens
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        B = ctx.B
+        s = torch.stack([torch.as_tensor(count_scale(B.O, N), device=B.dev) for N in B.Ns])
+        w = (grad_out.to(B.dev, torch.float64) * s)
+        g = B.adjoint(w)
+        return g.to(torch.float64).cpu(), None

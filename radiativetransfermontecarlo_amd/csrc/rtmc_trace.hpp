@@ -1,0 +1,773 @@
+// rtmc_trace.hpp -- the persistent tracing kernel behind rtmc_forward,
+// rtmc_forward_batched and rtmc_adjoint_batched.
+//
+// Replaces the packet x time loop of radiativeTransfer.m:40-158.  One lane
+// owns one particle from birth to the last observation time; there is no
+// particle state in HBM, only the histogram.
+//
+// Sub-stepped process (material.timeSteps = 0).  propagateParticleSmallDt
+// advances every particle by sub-steps of length dt_s and, in each sub-step:
+// removes it with probability pKill (Q attenuation), moves it, applies the
+// boundaries, and scatters it with probability 1 - exp(-dt_s/mft).  These
+// Bernoulli trials are independent, so the index of the next sub-step with a
+// scattering (or removal) is geometric; the kernel samples those indices
+// directly (G = ceil(E / lambda), E ~ Exp(1)) and jumps from event to event,
+// executing a sub-step in full only where something happens: a scattering, a
+// removal, or a predicted boundary crossing.  In between the particle flies
+// straight exactly as the sub-step loop would move it.  The sampled process
+// has the same law as the reference's; its cost is O(events + observations)
+// instead of O(sub-steps).  Observations happen at the end of sub-step
+// it * n_sub, after that sub-step's scattering (radiativeTransfer.m:72-77).
+//
+// Event-driven process (material.timeSteps = 1, propagateParticle.m): the
+// free flight between scatterings is exponential with mean mft(mode).
+//
+// A launch holds one or more problems (material + source + particle count +
+// outputs) sharing the geometry, the observation bins and the time axis.  A
+// work-group works on one problem at a time: it picks the problem with the
+// most estimated work left per work-group already on it, its wavefronts
+// draw particles of that problem from the problem's counter until none are
+// left, and it then picks again.  With one problem this is a plain
+// persistent kernel with dynamic particle distribution.
+//
+// Histogram (forward): LDS-privatised per work-group when the whole
+// histogram of a problem fits in LDS (flushed with global atomics when the
+// work-group leaves the problem), otherwise global no-return atomics.
+//
+// Adjoint: the same kernel replays every trajectory from its Philox counter
+// (same draws, same events, same deposits) and, at each deposit into bin b
+// at observation time t, adds weight[b] * S(t) to the gradient, where S(t)
+// is the score of the trajectory up to t with respect to (c, lc, Q):
+//  - scattering test of each sub-step, probability 1 - exp(-lam),
+//    lam = dt_s Sigma: after k sub-steps with n scatterings
+//    d log L / d log lam = n lam / expm1(lam) - (k - n) lam  (= A);
+//    Sigma scales as c^2 (every coefficient of variation proportional to c,
+//    MaterialClass.m:233-235), so dlam/dc = 2 lam / c and the angle law does
+//    not depend on c;
+//  - scattering angle theta with density sigma(theta) J(theta) / Sigma:
+//    d log f / d lc = dlogsigma_dlc(theta) - dlogSigma_dlc, plus A dlogSigma_dlc
+//    from the rate;
+//  - Q removal test of each sub-step, survival exp(-omega dt_s / Q): after k
+//    sub-steps alive, d log L / dQ = k omega dt_s / Q^2 (the pathwise
+//    derivative of the survival weight, propagateParticleSmallDt.m:64-76).
+// Event-driven: A = n - t / mft (Poisson process of rate 1/mft), no Q.
+#pragma once
+
+#include "rtmc_common.hpp"
+
+namespace rtmc {
+
+constexpr int64_t kInf = (int64_t)1 << 60;
+
+// One problem of a launch, in device memory.
+struct ProbDev {
+  MatDev M;
+  rtmc_source S;
+  int64_t n;          // particles
+  uint64_t pid0;      // global id of particle 0
+  uint32_t k0, k1;    // Philox key
+  int64_t nsub;       // sub-steps per observation interval
+  double dt_sub;
+  double lamS[2];     // scattering: dt_s / mft(mode) (per sub-step)
+  double lamK[2];     // removal: omega dt_s / Q(mode) (per sub-step)
+  int useQ;
+  double cost;        // estimated work per particle (scheduling weight)
+  uint32_t* E;        // histogram (forward)
+  unsigned long long* counters;  // RTMC_NCOUNTERS or null
+  // adjoint
+  const float* w;     // dL/dE per bin
+  double* grad;       // [3]
+  double* dhist;      // [3][hist] or null
+  int64_t dhist_stride;
+  double two_over_c;  // d log lam / dc
+  double lam_over_expm1;  // lamS[0] / expm1(lamS[0])
+  double dlogS;       // d log Sigma / d lc
+  double kq;          // omega dt_s / Q^2 (0 when Q is infinite)
+  const double* glc;  // d log sigma(theta) / d lc at theta_k = k pi / (n_glc - 1)
+  int64_t n_glc;
+};
+
+struct LaunchParams {
+  rtmc_geometry G;
+  const double* edges[4];
+  int ne[4];
+  int eoff[4];        // offset of each edge array in the LDS copy
+  int64_t stride[4];
+  unsigned present;   // variables that exist (bit v; z is absent in 2-D)
+  int64_t time_stride;
+  int64_t mode_stride;
+  int n_times;
+  double dt_obs;
+  int edges_total;    // doubles of the LDS edge copy (0: edges read from global memory)
+  int chunk;          // particles fetched per counter access (per wavefront)
+  int batch;          // lanes waiting for an event before a wavefront takes an event step
+  int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
+  int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
+  int64_t hist_size;  // words of the LDS histogram / weight table (0 = global memory)
+  int n_prob;
+  unsigned long long* work;  // per-problem particle counters (zeroed)
+  unsigned int* active;      // per-problem count of work-groups on it (zeroed)
+};
+
+// Per-particle Philox stream: counter = {c, 0, pid_lo, pid_hi}, key = (k0, k1).
+struct Rng {
+  uint32_t k0, k1, plo, phi;
+  uint32_t c;
+  bool has;
+  double spare;
+  U4 w32;
+  int n32;
+  __device__ __forceinline__ void start(uint64_t pid) {
+    plo = (uint32_t)pid;
+    phi = (uint32_t)(pid >> 32);
+    c = 0u;
+    has = false;
+    n32 = 0;
+  }
+  __device__ __forceinline__ double next() {
+    if (has) {
+      has = false;
+      return spare;
+    }
+    U4 in;
+    in.x = c;
+    in.y = 0u;
+    in.z = plo;
+    in.w = phi;
+    U4 r = philox10(in, k0, k1);
+    c += 1u;
+    spare = u53(r.z, r.w);
+    has = true;
+    return u53(r.x, r.y);
+  }
+  // uniform in (0, 1] with 32 random bits (FP32 flight): one Philox word each
+  __device__ __forceinline__ double next32() {
+    if (n32 == 0) {
+      U4 in;
+      in.x = c;
+      in.y = 0u;
+      in.z = plo;
+      in.w = phi;
+      w32 = philox10(in, k0, k1);
+      c += 1u;
+      n32 = 4;
+    }
+    uint32_t w = (n32 == 4) ? w32.x : (n32 == 3) ? w32.y : (n32 == 2) ? w32.z : w32.w;
+    n32 -= 1;
+    return ((double)w + 1.0) * (1.0 / 4294967296.0);
+  }
+  template <typename R>
+  __device__ __forceinline__ double u() {
+    if (sizeof(R) == 8) return next();
+    return next32();
+  }
+};
+
+// index of the next sub-step with a success, trials of probability 1-exp(-lam)
+template <typename R>
+__device__ __forceinline__ int64_t geom_after(Rng& r, double lam) {
+  double u = r.u<R>();
+  if (!(lam > 0.0)) return kInf;
+  double e = (sizeof(R) == 8) ? -log(u) : (double)(-logf((float)u));
+  double g = ceil(e / lam);
+  if (g < 1.0) g = 1.0;
+  if (g > 1e15) return kInf;
+  return (int64_t)g;
+}
+
+// First sub-step (relative to the segment base) at which the straight
+// segment x = xb + n h d may trigger boundary B; a prediction that is at most
+// one sub-step early (the sub-step is then executed in full and found clear).
+template <typename R>
+__device__ __forceinline__ int64_t predict_boundary(const rtmc_boundary& B, const R xb_[3], const R d_[3], R h_) {
+  const double xb[3] = {xb_[0], xb_[1], xb_[2]};
+  const double d[3] = {d_[0], d_[1], d_[2]};
+  const double h = h_;
+  if (B.dir < 4) {
+    int b = B.dir - 1;
+    double s0 = xb[b] - B.val;
+    double r = h * d[b];
+    if (!(s0 * r < 0.0)) return kInf;  // moving away, parallel, or on the plane
+    double ns = fabs(s0) / fabs(r);
+    if (!(ns < 1e15)) return kInf;
+    int64_t nstar = (int64_t)floor(ns) + 1;
+    return nstar > 1 ? nstar - 1 : 1;
+  }
+  double ax = xb[0], ay = xb[1];
+  double bx = h * d[0], by = h * d[1];
+  double a2 = ax * ax + ay * ay - B.val * B.val;
+  if (a2 > 0.0) return 1;
+  double bb = bx * bx + by * by;
+  if (!(bb > 0.0)) return kInf;
+  double ab = ax * bx + ay * by;
+  double disc = ab * ab - bb * a2;
+  if (disc < 0.0) disc = 0.0;
+  double np = (-ab + sqrt(disc)) / bb;
+  if (!(np < 1e15)) return kInf;
+  int64_t f = (int64_t)floor(np);
+  return f > 1 ? f : 1;
+}
+
+// Binning of the observation variables, set up once per work-group from the
+// edges: which variables need a range check / bin search, and the mean
+// spacing used for the first guess of the bin.
+struct BinSetup {
+  unsigned check;  // variables to bin / range-check (bit v)
+  double e0[4];
+  double invw[4];
+};
+
+// Is a one-bin variable's range check always passed?  (then it is skipped)
+__device__ __forceinline__ bool trivially_inside(int v, int dim, int frame, double lo, double hi) {
+  if (v == 3) {  // direction: dir_z clamped to [-1,1] (3-D) or acos of it in [0, pi] (2-D)
+    return dim == 3 ? (lo <= -1.0 && hi >= 1.0) : (lo <= 0.0 && hi >= kPi);
+  }
+  if (frame == RTMC_FRAME_CARTESIAN || (v == 2 && frame == RTMC_FRAME_CYLINDRICAL)) {
+    return lo == -INFINITY && hi == INFINITY;  // a Cartesian coordinate
+  }
+  if (v == 0) return lo <= 0.0 && hi == INFINITY;  // radius
+  if (v == 1) return lo <= -kPi && hi >= kPi;      // atan2 azimuth
+  return lo <= -1.0 && hi >= 1.0;                  // sin(elevation)
+}
+
+__device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const double* sedges, BinSetup* bs) {
+  bs->check = 0u;
+  for (int v = 0; v < 4; ++v) {
+    bs->e0[v] = 0.0;
+    bs->invw[v] = 0.0;
+    if (!(p.present & (1u << v))) continue;
+    const double* e = sedges ? sedges + p.eoff[v] : p.edges[v];
+    double lo = e[0], hi = e[p.ne[v] - 1];
+    if (p.ne[v] == 2 && trivially_inside(v, dim, p.G.frame, lo, hi)) continue;
+    bs->check |= 1u << v;
+    bs->e0[v] = lo;
+    double w = hi - lo;
+    bs->invw[v] = (isfinite(w) && w > 0.0) ? (double)(p.ne[v] - 1) / w : 0.0;
+  }
+}
+
+// Birth (initializeParticle.m:17-147), computed in FP64 for both flight
+// precisions (once per particle; keeps dir_z = cos(pi/2) of 2-D births at its
+// FP64 value).
+template <int DIM>
+__device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x[3], double d[3], double q[3]) {
+  if (S.type == RTMC_SRC_POINT) {
+    double r;
+    if (S.radial.n == 0) {
+      double u1 = rng.next(), u2 = rng.next();
+      double g = sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2);
+      r = fabs(g * S.lambda);  // initializeParticle.m:23
+    } else {
+      r = table_eval(S.radial.x, S.radial.y, S.radial.n, S.radial.extrap, rng.next());  // :25-27
+    }
+    double phi0, phi;
+    if (DIM == 2) {
+      phi0 = kPi / 2;
+      phi = kPi / 2;
+    } else {
+      phi0 = acos(-1.0 + 2.0 * rng.next());
+      phi = acos(-1.0 + 2.0 * rng.next());
+    }
+    double theta0 = kTwoPi * rng.next();
+    double theta = kTwoPi * rng.next();
+    if (S.direction == RTMC_DIR_OUTGOING) {
+      theta = theta0;
+      phi = phi0;
+    } else if (S.direction == RTMC_DIR_UPPER) {
+      theta = kTwoPi * rng.next();
+      phi = acos(rng.next());
+    }
+    double st, ct, sp, cp;
+    sincos(theta, &st, &ct);
+    sincos(phi, &sp, &cp);
+    d[0] = ct * sp;
+    d[1] = st * sp;
+    d[2] = cp;
+    q[0] = -st;
+    q[1] = ct;
+    q[2] = 0.0;
+    if (DIM == 3) {
+      double alpha = kTwoPi * rng.next();
+      double c[3];
+      cross3(d, q, c);
+      double sa, ca;
+      sincos(alpha, &sa, &ca);
+      for (int k = 0; k < 3; ++k) q[k] = q[k] * ca + c[k] * sa;
+    }
+    double st0, ct0, sp0, cp0;
+    sincos(theta0, &st0, &ct0);
+    sincos(phi0, &sp0, &cp0);
+    x[0] = S.position[0] + r * (ct0 * sp0);
+    x[1] = S.position[1] + r * (st0 * sp0);
+    x[2] = S.position[2] + r * cp0;
+  } else {
+    double u1 = rng.next(), u2 = rng.next();
+    double g = sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2);
+    double c_long = g * S.lambda / 2.0;
+    double c1, c2;
+    if (S.aperture == RTMC_APERTURE_RECTANGLE) {
+      c1 = (rng.next() - 0.5) * S.extent[0];
+      c2 = (rng.next() - 0.5) * S.extent[1];
+    } else {
+      double rho = S.extent[0] * sqrt(rng.next());
+      double alpha = kTwoPi * rng.next();
+      double sa, ca;
+      sincos(alpha, &sa, &ca);
+      c1 = rho * ca;
+      c2 = rho * sa;
+    }
+    for (int k = 0; k < 3; ++k)
+      x[k] = ((S.position[k] + c_long * S.normal[k]) + c1 * S.basis_u[k]) + c2 * S.basis_v[k];
+    for (int k = 0; k < 3; ++k) d[k] = S.normal[k];
+    double th = rng.next() * kTwoPi;
+    double s, c;
+    sincos(th, &s, &c);
+    for (int k = 0; k < 3; ++k) q[k] = c * S.basis_u[k] + s * S.basis_v[k];
+  }
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void birth(const rtmc_source& S, Rng& rng, R xo[3], R dd[3], R qo[3]) {
+  double x[3], d[3], q[3];
+  birth64<DIM>(S, rng, x, d, q);
+  for (int k = 0; k < 3; ++k) {
+    xo[k] = (R)x[k];
+    dd[k] = (R)d[k];
+    qo[k] = (R)q[k];
+  }
+}
+
+// Histogram index of an observation (observeTime.m:15-45 + histcounts), or -1.
+template <typename R>
+__device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSetup& bs,
+                                             const double* __restrict__ sedges, int dim, const R X[3], R dz,
+                                             int isP, int it) {
+  double v[4];
+  obs_vars(dim, p.G.frame, X, dz, bs.check, v);
+  int64_t idx = (int64_t)it * p.time_stride;
+  if (p.edges_total > 0) {
+    // edges in LDS (`sedges` points into the work-group's LDS block)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!(bs.check & (1u << k))) continue;
+      int b = bin_guess(sedges + p.eoff[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
+      if (b < 0) return -1;
+      idx += (int64_t)b * p.stride[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!(bs.check & (1u << k))) continue;
+      int b = bin_guess(p.edges[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
+      if (b < 0) return -1;
+      idx += (int64_t)b * p.stride[k];
+    }
+  }
+  if (!isP) idx += p.mode_stride;
+  return idx;
+}
+
+// d log sigma / d lc at theta (linear interpolation on the uniform grid)
+__device__ __forceinline__ double glc_eval(const ProbDev& P, double th) {
+  const double h = kPi / (double)(P.n_glc - 1);
+  double s = th / h;
+  if (!(s > 0.0)) s = 0.0;
+  const double smax = (double)(P.n_glc - 1);
+  if (s > smax) s = smax;
+  int64_t i = (int64_t)s;
+  if (i > P.n_glc - 2) i = P.n_glc - 2;
+  double f = s - (double)i;
+  return P.glc[i] + f * (P.glc[i + 1] - P.glc[i]);
+}
+
+// Problem choice of a work-group: most estimated work left per work-group on
+// it (+1 for the one choosing).  Returns -1 when every problem is handed out.
+__device__ __forceinline__ int pick_problem(const LaunchParams& p, const ProbDev* __restrict__ probs) {
+  int best = -1;
+  double bestv = 0.0;
+  for (int i = 0; i < p.n_prob; ++i) {
+    unsigned long long w = __hip_atomic_load(p.work + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int64_t n = probs[i].n;
+    if ((int64_t)w >= n) continue;
+    unsigned a = __hip_atomic_load(p.active + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double v = (double)(n - (int64_t)w) * probs[i].cost / (double)(a + 1u);
+    if (v > bestv) {
+      bestv = v;
+      best = i;
+    }
+  }
+  if (best >= 0) atomicAdd(p.active + best, 1u);
+  return best;
+}
+
+// SINK: 0 forward histogram, 1 adjoint (weighted scores)
+// LDS-histogram kernels run one 1024-lane work-group per CU (the histogram
+// takes most of the LDS); the others use 256-lane work-groups.
+template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND, typename R, int SINK>
+__global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams p,
+                                                             const ProbDev* __restrict__ probs) {
+  extern __shared__ double smem[];
+  double* sedges = smem;
+  __shared__ BinSetup s_bs;
+  __shared__ double s_tx0[4], s_tx1[4];
+  __shared__ int s_cur;
+  int* sguide = (int*)(smem + p.edges_total);
+  uint32_t* shist = (uint32_t*)(sguide + (p.guide_n > 0 ? p.ntab * (p.guide_n + 1) : 0));
+  float* sw = (float*)shist;  // adjoint: the weight table takes the histogram's place
+  constexpr bool ADJ = SINK == 1;
+
+  if (p.edges_total > 0) {
+    // gather the edge arrays into one LDS block
+    for (int v = 0; v < 4; ++v) {
+      if (!(p.present & (1u << v))) continue;
+      for (int k = threadIdx.x; k < p.ne[v]; k += blockDim.x) smem[p.eoff[v] + k] = p.edges[v][k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) bin_setup(p, DIM, sedges, &s_bs);
+  __syncthreads();
+  const BinSetup bs = s_bs;
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  while (true) {
+    if (threadIdx.x == 0) s_cur = pick_problem(p, probs);
+    __syncthreads();
+    const int cur = __builtin_amdgcn_readfirstlane(s_cur);
+    if (cur < 0) break;
+    const ProbDev& P = probs[cur];
+    const MatDev& M = P.M;
+    unsigned long long* const work = p.work + cur;
+
+    if (p.guide_n > 0) {
+      // guide table: cell c holds the largest knot index k <= n-2 with x[k] <= c/G
+      // (Chen & Asau's guide for inversion of a tabulated CDF)
+      const int G = p.guide_n;
+      for (int t = 0; t < p.ntab; ++t) {
+        const TableDev& T = M.inv[t];
+        if (threadIdx.x == 0) {
+          s_tx0[t] = T.x[0];
+          s_tx1[t] = T.x[T.n - 1];
+        }
+        for (int c = threadIdx.x; c <= G; c += blockDim.x) {
+          double q = (double)c / (double)G;
+          int64_t lo = 0, hi = T.n - 1;
+          if (q >= T.x[0]) {
+            while (hi - lo > 1) {
+              int64_t mid = lo + ((hi - lo) >> 1);
+              if (T.x[mid] <= q) lo = mid;
+              else hi = mid;
+            }
+          }
+          sguide[t * (G + 1) + c] = (int)lo;
+        }
+      }
+    }
+    if (LDSH) {
+      if (ADJ) {
+        for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) sw[k] = P.w[k];
+      } else {
+        for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) shist[k] = 0u;
+      }
+    }
+    __syncthreads();
+
+    // inverse-CDF lookup of the reference table (same interval and formula as
+    // table_eval), bracketed by the LDS guide when present
+    auto lookup = [&](int t, double u) -> double {
+      const TableDev& T = M.inv[t];
+      if (p.guide_n > 0 && u >= s_tx0[t] && u <= s_tx1[t]) {
+        const int G = p.guide_n;
+        int c = (int)(u * (double)G);
+        c = c < 0 ? 0 : (c > G - 1 ? G - 1 : c);
+        const int* g = sguide + t * (G + 1);
+        return table_eval_bracket(T.x, T.y, T.n, T.extrap, u, g[c], (int64_t)g[c + 1] + 1);
+      }
+      return table_eval(T.x, T.y, T.n, T.extrap, u);
+    };
+    unsigned long long nscat = 0, ndep = 0, nbnd = 0, nkill = 0;
+    double ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;  // adjoint: dL/dc, dL/dlc, dL/dQ of this lane
+    // ---- lane state machine ------------------------------------------------
+    // Each lane holds one particle and its own next observation index.  One
+    // iteration is either a DEPOSIT step (every flying lane deposits its next
+    // observation) or an EVENT step (lanes whose next event comes before that
+    // observation process it, idle lanes take a new particle).  The wavefront
+    // takes an event step only when at least `batch` lanes wait for one (or no
+    // lane can fly), so the expensive event path runs for many lanes at once
+    // instead of once per observation for whichever lane happens to need it.
+    Rng rng;
+    rng.k0 = P.k0;
+    rng.k1 = P.k1;
+    rng.start(0);
+    rng.spare = 0.0;
+    R xb[3] = {R(0), R(0), R(0)}, d[3] = {R(0), R(0), R(1)}, q[3] = {R(1), R(0), R(0)};
+    int isP = 1;
+    R h = R(0);                     // sub-stepped: step length v dt_s; event-driven: speed v
+    int64_t kb = 0, Ks = kInf, Kk = kInf, kev = kInf;   // sub-stepped: base, next scatter / removal / event
+    double tb = 0.0, ts = INFINITY;                     // event-driven: base time, next scattering time
+    int it = 1;                     // next observation index
+    bool has = false;               // lane holds a particle
+    bool need = true;               // lane wants a particle
+    int nsc = 0;                    // adjoint: scatterings of the particle so far
+    double gsum = 0.0;              // adjoint: sum of dlogsigma_dlc(theta) over them
+    // wave-uniform pool of particle indices fetched from the problem's counter
+    int64_t pool_next = 0, pool_end = 0;
+    bool exhausted = false;
+
+    auto next_event = [&]() {
+      // sub-stepped: next sub-step at which something happens to this particle
+      int64_t kbd = kInf;
+      if (BND) {
+        for (int ib = 0; ib < p.G.n_bnd; ++ib) {
+          int64_t nb = predict_boundary(p.G.bnd[ib], xb, d, h);
+          if (nb < kInf) kbd = min(kbd, kb + nb);
+        }
+      }
+      kev = min(Ks, min(Kk, kbd));
+    };
+
+    while (true) {
+      const bool pend_ev = has && (QUANT ? (kev <= (int64_t)it * P.nsub) : (ts <= (double)it * p.dt_obs));
+      const bool pend = need || pend_ev;
+      const bool fly = has && !pend_ev;
+      const uint64_t pm = __ballot(pend);
+      const uint64_t fm = __ballot(fly);
+      if (pm == 0ull && fm == 0ull) break;
+      if (__popcll(pm) >= p.batch || fm == 0ull) {
+        // ---------------- EVENT step ----------------
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+          // hand out particle indices to the lanes that need one
+          const int n = __popcll(nm);
+          int64_t L = pool_end - pool_next;
+          int64_t nb0 = 0, nb1 = 0;
+          if (L < n && !exhausted) {
+            unsigned long long base = 0;
+            if (lane == __ffsll((long long)nm) - 1) base = atomicAdd(work, (unsigned long long)p.chunk);
+            base = __shfl(base, __ffsll((long long)nm) - 1, 64);
+            if ((int64_t)base >= P.n) {
+              exhausted = true;
+            } else {
+              nb0 = (int64_t)base;
+              nb1 = min((int64_t)base + (int64_t)p.chunk, P.n);
+            }
+          }
+          if (need) {
+            const int r = __popcll(nm & lt_mask);
+            int64_t id = -1;
+            if (r < L) id = pool_next + r;
+            else if (r - L < nb1 - nb0) id = nb0 + (r - L);
+            if (id >= 0) {
+              rng.start(P.pid0 + (uint64_t)id);
+              birth<DIM>(P.S, rng, xb, d, q);
+              isP = (!AC && P.S.polarization == 1) ? 0 : 1;
+              it = 1;
+              has = true;
+              need = false;
+              nsc = 0;
+              gsum = 0.0;
+              if (QUANT) {
+                h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
+                kb = 0;
+                Ks = geom_after<R>(rng, P.lamS[isP ? 0 : 1]);
+                Kk = P.useQ ? geom_after<R>(rng, P.lamK[isP ? 0 : 1]) : kInf;
+                next_event();
+              } else {
+                h = (R)(isP ? M.vp : M.vs);
+                tb = 0.0;
+                ts = -log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
+              }
+            } else if (exhausted || r - L >= nb1 - nb0) {
+              need = false;  // nothing left to trace
+            }
+          }
+          if (n <= L) {
+            pool_next += n;
+          } else if (nb1 > nb0) {
+            pool_next = nb0 + min((int64_t)(n - L), nb1 - nb0);
+            pool_end = nb1;
+          } else {
+            pool_next = pool_end;
+          }
+        }
+        if (pend_ev) {
+          if (QUANT) {
+            // sub-step j = kev in full (propagateParticleSmallDt.m:85-642); straight
+            // flight from the base to the end of sub-step j-1 first
+            const int64_t j = kev;
+            const R m = (R)(j - 1 - kb);
+            R x[3];
+            for (int k = 0; k < 3; ++k) x[k] = xb[k] + (m * h) * d[k];
+            bool alive = true;
+            if (Kk == j) {  // removed at the start of sub-step j (:100-138)
+              alive = false;
+              nkill += 1;
+            } else {
+              R x1[3] = {x[0], x[1], x[2]};
+              for (int k = 0; k < 3; ++k) x[k] = x[k] + h * d[k];
+              const int wasP = isP;
+              for (int ib = 0; BND && ib < p.G.n_bnd; ++ib) {
+                const rtmc_boundary& B = p.G.bnd[ib];
+                bool hit = (B.dir < 4) ? plane_crossed(B, x, x1) : cyl_outside(B, x);
+                if (!hit) continue;
+                nbnd += 1;
+                if (B.type == RTMC_BND_ABSORBING) {
+                  alive = false;
+                  nkill += 1;
+                  break;
+                }
+                double uA = 0.0, uB = 0.0;
+                if (!AC) {
+                  uA = rng.u<R>();
+                  uB = rng.u<R>();
+                }
+                if (B.dir < 4) plane_reflect(M, AC, B, x, d, q, isP, uA, uB);
+                else cyl_reflect(M, AC, B, x, d, q, isP, uA, uB);
+              }
+              if (alive) {
+                if (!AC && isP != wasP) {
+                  // mode changed at the boundary: this sub-step's scattering test and
+                  // the removal tests from the next one use the new mode
+                  h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
+                  Ks = j - 1 + geom_after<R>(rng, P.lamS[isP ? 0 : 1]);
+                  Kk = P.useQ ? j + geom_after<R>(rng, P.lamK[isP ? 0 : 1]) : kInf;
+                }
+                if (Ks == j) {
+                  const int before = isP;
+                  double uphi = rng.u<R>();
+                  double uth = rng.u<R>();
+                  double uconv = AC ? 0.0 : rng.u<R>();
+                  double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
+                  if (ADJ) {
+                    nsc += 1;
+                    gsum += glc_eval(P, th);
+                  }
+                  rotate_renorm(d, q, (R)th);
+                  nscat += 1;
+                  if (!AC && isP != before) {
+                    h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
+                    if (P.useQ) Kk = j + geom_after<R>(rng, P.lamK[isP ? 0 : 1]);
+                  }
+                  Ks = j + geom_after<R>(rng, P.lamS[isP ? 0 : 1]);
+                }
+                xb[0] = x[0];
+                xb[1] = x[1];
+                xb[2] = x[2];
+                kb = j;
+                next_event();
+              }
+            }
+            if (!alive) {
+              has = false;
+              need = true;
+            }
+          } else {
+            // event-driven: scattering at time ts (propagateParticle.m:24-72)
+            const R f = (R)((double)h * (ts - tb));
+            for (int k = 0; k < 3; ++k) xb[k] = xb[k] + f * d[k];
+            tb = ts;
+            double uphi = rng.u<R>();
+            double uth = rng.u<R>();
+            double uconv = AC ? 0.0 : rng.u<R>();
+            double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
+            if (ADJ) {
+              nsc += 1;
+              gsum += glc_eval(P, th);
+            }
+            rotate_renorm(d, q, (R)th);
+            nscat += 1;
+            h = (R)(isP ? M.vp : M.vs);
+            ts = tb - log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
+          }
+        }
+      } else {
+        // ---------------- DEPOSIT step ----------------
+        if (fly) {
+          R X[3];
+          if (QUANT) {
+            const R m = (R)((int64_t)it * P.nsub - kb);
+            for (int k = 0; k < 3; ++k) X[k] = xb[k] + (m * h) * d[k];
+          } else {
+            const R f = (R)((double)h * ((double)it * p.dt_obs - tb));
+            for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
+          }
+          const int64_t idx = bin_index(p, bs, sedges, DIM, X, d[2], isP, it);
+          if (idx >= 0) {
+            ndep += 1;
+            if (!ADJ) {
+              if (LDSH) atomicAdd(shist + idx, 1u);
+              else __hip_atomic_fetch_add(P.E + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+              // score of the trajectory up to this observation
+              double A, sq;
+              if (QUANT) {
+                const double k = (double)it * (double)P.nsub;
+                const double lam = P.lamS[0];
+                A = (double)nsc * P.lam_over_expm1 - (k - (double)nsc) * lam;
+                sq = k * P.kq;
+              } else {
+                A = (double)nsc - (double)it * p.dt_obs / M.mftP;
+                sq = 0.0;
+              }
+              const double s0 = A * P.two_over_c;
+              const double s1 = A * P.dlogS + gsum - (double)nsc * P.dlogS;
+              const double w = LDSH ? (double)sw[idx] : (double)P.w[idx];
+              ga0 += w * s0;
+              ga1 += w * s1;
+              ga2 += w * sq;
+              if (P.dhist) {
+                unsafeAtomicAdd(P.dhist + idx, s0);
+                unsafeAtomicAdd(P.dhist + P.dhist_stride + idx, s1);
+                if (sq != 0.0) unsafeAtomicAdd(P.dhist + 2 * P.dhist_stride + idx, sq);
+              }
+            }
+          }
+          it += 1;
+          if (it >= p.n_times) {
+            has = false;
+            need = true;
+          }
+        }
+      }
+    }
+
+    __syncthreads();
+    if (LDSH && !ADJ) {
+      for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) {
+        uint32_t c = shist[k];
+        if (c) __hip_atomic_fetch_add(P.E + k, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (P.counters) {
+      // one atomic per wavefront and counter
+      for (int off = 32; off > 0; off >>= 1) {
+        nscat += __shfl_down(nscat, off, 64);
+        ndep += __shfl_down(ndep, off, 64);
+        nbnd += __shfl_down(nbnd, off, 64);
+        nkill += __shfl_down(nkill, off, 64);
+      }
+      if (lane == 0) {
+        if (nscat) atomicAdd(P.counters + RTMC_CNT_SCATTER, nscat);
+        if (ndep) atomicAdd(P.counters + RTMC_CNT_DEPOSIT, ndep);
+        if (nbnd) atomicAdd(P.counters + RTMC_CNT_BOUNDARY, nbnd);
+        if (nkill) atomicAdd(P.counters + RTMC_CNT_KILLED, nkill);
+      }
+    }
+    if (ADJ) {
+      for (int off = 32; off > 0; off >>= 1) {
+        ga0 += __shfl_down(ga0, off, 64);
+        ga1 += __shfl_down(ga1, off, 64);
+        ga2 += __shfl_down(ga2, off, 64);
+      }
+      if (lane == 0) {
+        unsafeAtomicAdd(P.grad + 0, ga0);
+        unsafeAtomicAdd(P.grad + 1, ga1);
+        unsafeAtomicAdd(P.grad + 2, ga2);
+      }
+    }
+    if (threadIdx.x == 0) atomicSub(p.active + cur, 1u);
+    __syncthreads();  // the LDS tables and s_cur are rewritten for the next problem
+  }
+}
+
+}  // namespace rtmc

@@ -1,0 +1,7 @@
+// rtmc_trace_3d_f32.hip -- k_trace instances for 3-D problems with float flight arithmetic.
+#include "rtmc_trace_launch.hpp"
+
+namespace rtmc {
+template int launch_trace<3, float>(const LaunchParams&, const ProbDev*, bool, bool, bool, bool, int, size_t,
+                                      unsigned, unsigned, hipStream_t);
+}  // namespace rtmc

@@ -402,8 +402,16 @@ class MaterialClass:
 
     # -- scattering tables (MaterialClass.m:2379-2463) -------------------
     @staticmethod
-    def prepareSigmaOne(sigma: Callable, d: int, Nth: int = 1_000_000):
-        """Sigma, Sigma', and the reference's inverse CDF of the scattering angle."""
+    def prepareSigmaOne(sigma: Callable, d: int, Nth: int = 1_000_000, knot_threshold: float = 1e-8):
+        """Sigma, Sigma', and the reference's inverse CDF of the scattering angle.
+
+        Knots are kept where the cdf step exceeds ``knot_threshold`` (the
+        reference's 1e-8, MaterialClass.m:2460) and the table extrapolates
+        linearly beyond them, so the sampled tail beyond the last knot is not
+        sigma's.  ``knot_threshold=0`` keeps every strictly increasing knot:
+        the table then follows sigma to within the 1e6-point grid everywhere,
+        which is what the adjoint's likelihood-ratio score assumes (beyond
+        the last knot it then holds the last angle)."""
         xth = np.linspace(0, np.pi, Nth)
         d = int(d)
         if d == 2:
@@ -420,18 +428,25 @@ class MaterialClass:
         if np.any(np.isnan(pdf)):
             warnings.warn('There is a NaN inside the probability density function')
         cdf = np.nancumsum(pdf) * (np.pi / Nth)
-        ind = np.nonzero(np.diff(cdf) > 1e-8)[0]
+        ind = np.nonzero(np.diff(cdf) > knot_threshold)[0]
         ind = np.unique(np.concatenate([ind, ind + 1]))
+        if knot_threshold <= 0:
+            # cdf steps that underflow next to 1 would repeat a knot; beyond the
+            # last knot (mass ~1/Nth) the table holds its last angle instead of
+            # extrapolating the near-vertical last segment
+            ind = ind[np.concatenate([[True], np.diff(cdf[ind]) > 0])]
+            return Sigma, Sigma_prime, InverseCDF(cdf[ind], xth[ind], extrap="nearest")
         invcdf = InverseCDF(cdf[ind], xth[ind])
         return Sigma, Sigma_prime, invcdf
 
     @staticmethod
-    def prepareSigma(mat: "MaterialClass", d: int) -> "MaterialClass":
+    def prepareSigma(mat: "MaterialClass", d: int, knot_threshold: float = 1e-8) -> "MaterialClass":
         if not mat.sigma:
             mat.CalcSigma()
         d = int(d)
         if mat.acoustics:
-            mat.Sigma, mat.Sigmapr, mat.invcdf = MaterialClass.prepareSigmaOne(mat.sigma[0], d)
+            mat.Sigma, mat.Sigmapr, mat.invcdf = MaterialClass.prepareSigmaOne(mat.sigma[0], d,
+                                                                                    knot_threshold=knot_threshold)
             mat.Diffusivity = mat.v ** 2 / (float(d) * (mat.Sigma - mat.Sigmapr))
             mat.meanFreeTime = 1 / mat.Sigma
             mat.meanFreePath = mat.v * mat.meanFreeTime
@@ -446,7 +461,8 @@ class MaterialClass:
         inv = [[None, None], [None, None]]
         for a in range(2):
             for b in range(2):
-                S[a, b], Sp[a, b], inv[a][b] = MaterialClass.prepareSigmaOne(mat.sigma[a][b], d)
+                S[a, b], Sp[a, b], inv[a][b] = MaterialClass.prepareSigmaOne(mat.sigma[a][b], d,
+                                                                                 knot_threshold=knot_threshold)
         mat.Sigma, mat.Sigmapr, mat.invcdf = S, Sp, inv
         mat.meanFreeTime = 1.0 / S.sum(axis=1)
         vpvs = np.array([mat.vp, mat.vs], dtype=np.float64)

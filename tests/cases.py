@@ -72,3 +72,19 @@ def config3(n=50_000_000):
     material.rho = 2000
     material.Q = math.inf
     return geometry, source, material, observation
+
+
+def config4(n_items=64, seed=20240604):
+    """64 3-D acoustic materials on the config1 geometry and observation:
+    cv = [c c], c ~ U[0.02, 0.15], lc ~ U[0.05, 0.5], Q ~ U[50, 1000], exp ACF,
+    f = 10, v = 1; particles per material log-uniform in [1e5, 1e7] (numpy
+    default_rng(seed)).  Returns (geometry, source, base material,
+    observation, params (n x 3: c, lc, Q), N (n))."""
+    geometry, source, base, observation = config1()
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.02, 0.15, n_items)
+    lc = rng.uniform(0.05, 0.5, n_items)
+    Q = rng.uniform(50.0, 1000.0, n_items)
+    N = np.round(10.0 ** rng.uniform(5.0, 7.0, n_items)).astype(np.int64)
+    params = np.stack([c, lc, Q], axis=1)
+    return geometry, source, base, observation, params, N

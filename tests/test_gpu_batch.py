@@ -1,0 +1,252 @@
+"""Batched forward and adjoint kernels (rtmc_forward_batched, rtmc_adjoint_batched).
+
+- A batch item gives exactly the histogram and counters of rtmc_forward with
+  the same material, source, particles and seed (same trajectories).
+- The adjoint replays the forward: same deposits and scatterings.
+- Q part of the adjoint: the score of surviving k sub-steps is k * omega
+  dt_s / Q^2 exactly, so dL/dQ = sum_b w_b count_b k(t_b) omega dt_s / Q^2.
+- The score has zero mean: with Q = Inf and every particle observed at t*,
+  sum_b count_b(t*) = N does not depend on (c, lc), and the estimate of its
+  derivative must vanish within Monte Carlo error.
+- Against common-random-number central finite differences of the forward.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from radiativetransfermontecarlo_amd import batch as rb
+from radiativetransfermontecarlo_amd import solver
+from radiativetransfermontecarlo_amd.material import MaterialClass
+from radiativetransfermontecarlo_amd.observation import initializeObservation
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = [(0.05, 0.1, math.inf), (0.10, 0.30, 200.0), (0.03, 0.05, 800.0), (0.15, 0.45, 60.0),
+          (0.08, 0.20, 300.0)]
+NS = [200_000, 50_000, 123_457, 80_000, 4_000]
+SEEDS = [5, 6, 7, 8, 9]
+
+
+def _acoustic(tmax=1.0, params=PARAMS):
+    geometry, source, base, observation = cases.config1()
+    observation = dict(observation, time=np.arange(0, tmax + 1e-9, 0.02))
+    mats = [rb.prepare_material(base, 3, c=c, lc=lc, Q=Q) for c, lc, Q in params]
+    O = initializeObservation(geometry, True, observation, 1)
+    return geometry, source, base, O, mats
+
+
+@pytest.mark.parametrize("hist_mode,precision", [(0, "fp64"), (1, "fp64"), (0, "fp32")])
+def test_batch_items_equal_single_forward(gpu, hist_mode, precision):
+    geometry, source, base, O, mats = _acoustic()
+    B = rb.Batch(geometry, O, mats, source, NS, SEEDS, device='cuda', precision=precision)
+    B.configure(hist_mode=hist_mode)
+    E, cnt = B.forward()
+    for i, m in enumerate(mats):
+        Es, cs, _k = solver.forward_histogram(geometry, source, m, O, NS[i], seed=SEEDS[i], device='cuda',
+                                              hist_mode=hist_mode, precision=precision)
+        assert torch.equal(E[i], Es), f"item {i}"
+        assert torch.equal(cnt[i], cs), f"item {i}"
+
+
+def test_batch_elastic_items_equal_single_forward(gpu):
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-3, 3, 25)
+    observation = {'x': e, 'y': e, 'time': np.arange(0, 0.8 + 1e-9, 0.05)}
+    mats = []
+    for Q in ([40.0, 20.0], [math.inf, math.inf]):
+        m = cases.elastic_iso(2)
+        m.Q = Q
+        m.Frequency = 1.0
+        mats.append(MaterialClass.prepareSigma(m, 2))
+    O = initializeObservation(geometry, False, observation, 1)
+    Ns, seeds = [70_000, 30_001], [3, 4]
+    B = rb.Batch(geometry, O, mats, source, Ns, seeds, device='cuda')
+    E, cnt = B.forward()
+    for i, m in enumerate(mats):
+        Es, cs, _k = solver.forward_histogram(geometry, source, m, O, Ns[i], seed=seeds[i], device='cuda')
+        assert torch.equal(E[i], Es) and torch.equal(cnt[i], cs), f"item {i}"
+
+
+def test_adjoint_replays_forward_and_q_identity(gpu):
+    geometry, source, base, O, mats = _acoustic()
+    B = rb.Batch(geometry, O, mats, source, NS, SEEDS, device='cuda')
+    E, cnt = B.forward()
+    Ef = E.cpu().numpy().view(np.uint32).astype(np.float64)
+    cnt_f = cnt.clone()
+    gen = torch.Generator().manual_seed(1)
+    w = (torch.rand((len(mats), O.size), generator=gen, dtype=torch.float32) * 2 - 1)
+    B.counters.zero_()
+    g = B.adjoint(w).cpu().numpy()
+    # the replay makes the same deposits and scatterings
+    assert torch.equal(B.counters, cnt_f)
+    dt = float(np.mean(np.diff(O.t)))
+    it = np.arange(O.size) // O.time_stride
+    for i, (c, lc, Q) in enumerate(PARAMS):
+        mft = float(mats[i].meanFreeTime)
+        dts = min(0.01 * mft, dt)
+        nsub = math.ceil(dt / dts)
+        dts = dt / nsub
+        kq = 2.0 * math.pi * 10.0 * dts / (Q * Q) if math.isfinite(Q) else 0.0
+        expect = float(np.sum(w[i].numpy().astype(np.float64) * Ef[i] * (it * nsub) * kq))
+        assert g[i, 2] == pytest.approx(expect, rel=1e-9, abs=1e-300), f"item {i}"
+
+
+def test_adjoint_score_has_zero_mean(gpu):
+    """d/d(c, lc) of the particle count at t* (all particles observed, Q = Inf) is 0."""
+    params = [(0.10, 0.2, math.inf)] * 8
+    geometry, source, base, O, mats = _acoustic(tmax=1.0, params=params)
+    N = 250_000
+    B = rb.Batch(geometry, O, mats, source, [N] * 8, list(range(100, 108)), device='cuda')
+    E, _ = B.forward()
+    tstar = O.Nt - 1
+    w = torch.zeros((8, O.size), dtype=torch.float32)
+    w[:, tstar * O.time_stride:(tstar + 1) * O.time_stride] = 1.0
+    assert int(E[0, tstar * O.time_stride:(tstar + 1) * O.time_stride].sum()) == N
+    g = B.adjoint(w).cpu().numpy()
+    for k in (0, 1):
+        m, s = g[:, k].mean(), g[:, k].std(ddof=1) / math.sqrt(8)
+        print(f"param {k}: mean {m:.4g} +- {s:.4g} (per-item {g[:, k]})")
+        assert abs(m) < 4.5 * s
+
+
+def _loss(dens, target, mask):
+    r = np.log(np.where(mask, dens, 1.0)) - np.log(np.where(mask, target, 1.0))
+    return float(np.sum(np.where(mask, r * r, 0.0)))
+
+
+def test_adjoint_matches_crn_finite_differences(gpu):
+    """Tables without the reference's knot truncation (knot_threshold=0): the
+    likelihood-ratio score differentiates sigma, which such tables follow."""
+    geometry, source, base, observation = cases.config1()
+    observation = dict(observation, time=np.arange(0, 2 + 1e-9, 0.02))
+    O = initializeObservation(geometry, True, observation, 1)
+    p0 = np.array([0.08, 0.2, 300.0])
+    N, seeds = 1_000_000, [21, 22, 23, 24]
+    scale = rb.count_scale(O, N)
+    # target: perturbed parameters, independent particles
+    pt = p0 * np.array([1.05, 0.95, 1.10])
+    Bt = rb.Batch(geometry, O, [rb.prepare_material(base, 3, *pt, knot_threshold=0.0)], source, [4 * N], [999], device='cuda')
+    Bt.forward()
+    target = Bt.E[0].cpu().numpy().view(np.uint32) * rb.count_scale(O, 4 * N)
+
+    def run(p, seed):
+        B = rb.Batch(geometry, O, [rb.prepare_material(base, 3, *p, knot_threshold=0.0)], source, [N], [seed],
+                     device='cuda')
+        B.forward()
+        cnt = B.E[0].cpu().numpy().view(np.uint32)
+        return B, cnt * scale, cnt
+
+    lr, fd = [], []
+    for seed in seeds:
+        B, dens, cnt = run(p0, seed)
+        # bins of the loss: target > 0 and enough counts that no perturbed run empties them
+        mask = (target > 0) & (cnt >= 16)
+        w = np.where(mask, 2 * (np.log(np.where(mask, dens, 1)) - np.log(np.where(mask, target, 1)))
+                     / np.where(mask, dens, 1), 0.0) * scale
+        g = B.adjoint(torch.as_tensor(w[None, :], dtype=torch.float32)).cpu().numpy()[0]
+        lr.append(g)
+        gf = []
+        for k, h in enumerate((0.02, 0.02, 0.05)):
+            dp = np.zeros(3)
+            dp[k] = h * p0[k]
+            Lp = _loss(run(p0 + dp, seed)[1], target, mask)
+            Lm = _loss(run(p0 - dp, seed)[1], target, mask)
+            gf.append((Lp - Lm) / (2 * dp[k]))
+        fd.append(gf)
+    lr, fd = np.array(lr), np.array(fd)
+    for k in range(3):
+        ml, mf = lr[:, k].mean(), fd[:, k].mean()
+        se = math.sqrt(lr[:, k].var(ddof=1) / len(seeds) + fd[:, k].var(ddof=1) / len(seeds))
+        print(f"param {k}: LR {ml:.5g} (per seed {lr[:, k]}), FD {mf:.5g} (per seed {fd[:, k]}), se {se:.3g}")
+        assert abs(ml - mf) < max(4 * se, 0.03 * abs(mf)), f"parameter {k}"
+        assert abs(ml - mf) < 0.1 * abs(mf), f"parameter {k}"
+
+
+def test_plane_wave_mean_depth_closed_form(gpu):
+    """Plane wave along +z from z = 0 (no longitudinal spread), Q finite: each
+    sub-step moves h = v dt_s, then scatters with p = 1 - exp(-dt_s/mft) and
+    E[cos theta] = g1 of the sampled table, so after k sub-steps
+    E[z 1(alive)] = exp(-k omega dt_s/Q) h (1 - a^k)/(1 - a), a = 1 - p (1 - g1).
+    Pins the forward (mean depth) and the adjoint (its derivatives in c, lc, Q)."""
+    c, lc, Q = 0.08, 0.2, 300.0
+    geometry, _s, base, _o = cases.config1()
+    geometry = {'dimension': 3, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'type': 'plane', 'direction': 3, 'aperture': 'rectangle', 'extent': [1, 1],
+              'position': [0, 0, 0], 'lambda': 0.0}
+    zedges = np.arange(-1.0005, 1.0005 + 1e-9, 0.001)
+    observation = {'x': [-1e6, 1e6], 'y': [-1e6, 1e6], 'z': zedges, 'directions': [0, np.pi],
+                   'time': np.arange(0, 1 + 1e-9, 0.02)}
+    O = initializeObservation(geometry, True, observation, 1)
+    m = rb.prepare_material(base, 3, c=c, lc=lc, Q=Q, knot_threshold=0.0)
+    K, N = 4, 1_000_000
+    B = rb.Batch(geometry, O, [m] * K, source, [N] * K, [31 + i for i in range(K)], device='cuda')
+    E, _ = B.forward()
+    # closed form on the host
+    dt = float(np.mean(np.diff(O.t)))
+    mft = float(m.meanFreeTime)
+    dts = min(0.01 * mft, dt)
+    nsub = math.ceil(dt / dts)
+    dts = dt / nsub
+    it = O.Nt - 1
+    k = it * nsub
+    lam = dts / mft
+    p = -math.expm1(-lam)
+    u = (np.arange(1_000_000) + 0.5) / 1_000_000
+    g1 = float(np.cos(m.invcdf(u)).mean())
+    hlc = 1e-3 * lc
+    dg1 = (np.cos(rb.prepare_material(base, 3, c=c, lc=lc + hlc, Q=Q, knot_threshold=0.0).invcdf(u)).mean()
+           - np.cos(rb.prepare_material(base, 3, c=c, lc=lc - hlc, Q=Q, knot_threshold=0.0).invcdf(u)).mean()) \
+        / (2 * hlc)
+    g, dS = rb.lc_derivatives(m, 3)
+    h = 1.0 * dts
+    a = 1 - p * (1 - g1)
+    F = h * (1 - a ** k) / (1 - a)
+    dF = h * (-(k * a ** (k - 1)) * (1 - a) + (1 - a ** k)) / (1 - a) ** 2
+    omega = 2 * math.pi * 10
+    S = math.exp(-k * omega * dts / Q)
+    dp_dlam = math.exp(-lam)
+    da_dc = -(1 - g1) * dp_dlam * lam * 2 / c
+    da_dlc = -(1 - g1) * dp_dlam * lam * dS + p * dg1
+    expect = np.array([N * S * dF * da_dc, N * S * dF * da_dlc, N * F * S * k * omega * dts / Q ** 2])
+    zc = 0.5 * (zedges[1:] + zedges[:-1])
+    Wz = np.zeros(O.size)
+    nz = zc.size
+    sl = slice(it * O.time_stride, (it + 1) * O.time_stride)
+    Wz[sl] = np.tile(zc, O.time_stride // nz)
+    cnt = E.cpu().numpy().view(np.uint32).astype(np.float64)
+    zsum = cnt @ Wz
+    z2 = cnt @ (Wz ** 2)
+    alive = cnt[:, sl].sum(axis=1)
+    var = z2 / alive - (zsum / alive) ** 2
+    se_f = math.sqrt(np.mean(var) * N) / math.sqrt(K)
+    print(f"forward: sum z {zsum.mean():.6g} +- {se_f:.3g}, closed form {N * S * F:.6g}")
+    assert abs(zsum.mean() - N * S * F) < 4 * se_f + 1e-4 * N * S * F
+    w = torch.as_tensor(np.tile(Wz, (K, 1)), dtype=torch.float32)
+    gr = B.adjoint(w, tables=[(c, dS, g)] * K).cpu().numpy()
+    for j, name in enumerate(("c", "lc", "Q")):
+        mean, se = gr[:, j].mean(), gr[:, j].std(ddof=1) / math.sqrt(K)
+        print(f"d/d{name}: LR {mean:.6g} +- {se:.3g} (per item {gr[:, j]}), closed form {expect[j]:.6g}")
+        assert abs(mean - expect[j]) < 4 * se + 2e-3 * abs(expect[j]), name
+
+
+def test_histograms_do_not_depend_on_launch_configuration(gpu):
+    """Trajectories are keyed by (seed, particle id): block size, particles per
+    thread, histogram placement and event batch change only the schedule."""
+    geometry, source, base, O, mats = _acoustic()
+    B = rb.Batch(geometry, O, mats, source, NS, SEEDS, device='cuda')
+    E0 = B.forward()[0].clone()
+    for cfg in ({'block_size': 512, 'hist_mode': 2, 'event_batch': 8},
+                {'block_size': 128, 'hist_mode': 1, 'particles_per_thread': 4, 'event_batch': 32},
+                {'block_size': 1024, 'hist_mode': 2, 'particles_per_thread': 2, 'event_batch': 1}):
+        B.configure(**cfg)
+        assert torch.equal(B.forward()[0], E0), cfg
+    B.configure()
+    t = B.autotune(sample_fraction=0.05)
+    print("autotune:", t)
+    assert t["ms"] > 0 and t["ms"] <= t["ms_default"] * 1.0001
+    assert t["block_size"] in (128, 256, 512, 1024) and t["hist_mode"] in (1, 2)
+    assert torch.equal(B.forward()[0], E0)
