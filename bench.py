@@ -39,7 +39,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, choices=[1, 2], default=2)
+    ap.add_argument("--config", type=int, choices=[1, 2, 4], default=2)
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="config 4: multiply every material's particle count (1 = the configuration's own)")
+    ap.add_argument("--autotune", action="store_true", help="config 4: tune the batched launch first")
     ap.add_argument("--particles", type=int, default=0, help="0 = the configuration's own count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -90,6 +93,8 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.config == 4:
+        return run_batch_adjoint(args)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.int32, device="cuda")
@@ -184,6 +189,138 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, geometry, source, material, O)
+    print(json.dumps(out), flush=True)
+
+
+def run_batch_adjoint(args):
+    """64 acoustic materials: batched forward + adjoint against 64 plain forward
+    solves with central finite differences (7 solves per material)."""
+    import numpy as np
+    import torch
+    from radiativetransfermontecarlo_amd import batch as rb
+    from radiativetransfermontecarlo_amd import solver
+    from radiativetransfermontecarlo_amd.observation import initializeObservation
+    from tests import cases
+
+    dev = torch.device("cuda", 0)
+    geometry, source, base, observation, params, Ns = cases.config4()
+    Ns = np.maximum(1000, np.round(Ns * args.batch_scale)).astype(np.int64)
+    n = len(Ns)
+    O = initializeObservation(geometry, True, observation, 1)
+    KT = 0.0  # tables that follow sigma: the score differentiates sigma
+    H = np.array([0.02, 0.02, 0.05])  # central-difference steps (relative)
+    scales = torch.stack([torch.as_tensor(rb.count_scale(O, int(N)), device=dev) for N in Ns])
+
+    def sync():
+        torch.cuda.synchronize()
+
+    # target: perturbed parameters, independent particles (not timed)
+    pt = params * np.array([1.05, 0.95, 1.10])
+    Bt = rb.Batch(geometry, O, [rb.prepare_material(base, 3, *p, knot_threshold=KT) for p in pt], source,
+                  list(Ns), [7_000 + i for i in range(n)], device=dev)
+    Bt.forward()
+    target = Bt.energy_density()
+    del Bt
+    seeds = [1_000 + i for i in range(n)]
+
+    # ---- batched forward + adjoint -------------------------------------
+    sync()
+    t0 = time.perf_counter()
+    mats = [rb.prepare_material(base, 3, *p, knot_threshold=KT) for p in params]
+    B = rb.Batch(geometry, O, mats, source, list(Ns), seeds, device=dev)
+    tabs = B.derivative_tables()
+    t_prep_a = time.perf_counter() - t0
+    tuning = B.autotune(0.02) if args.autotune else None
+    sync()
+    t1 = time.perf_counter()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    B.forward()
+    e1.record()
+    dens = B.E.to(torch.float64) * scales
+    mask = (target > 0) & (B.E >= 16)
+    lg = torch.where(mask, torch.log(torch.where(mask, dens, 1.0)) - torch.log(torch.where(mask, target, 1.0)), 0.0)
+    loss = (lg * lg).sum(dim=1)
+    w = torch.where(mask, 2.0 * lg / torch.where(mask, dens, 1.0), 0.0) * scales
+    grad = B.adjoint(w, tables=tabs)
+    e2.record()
+    sync()
+    t_a = time.perf_counter() - t1
+    g_lr = grad.cpu().numpy()
+    ms_fwd, ms_adj = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    nscat = int(B.counters[:, 0].sum())
+    ndep = int(B.counters[:, 1].sum())
+    mask_np = mask.cpu().numpy()
+
+    # ---- plain forward solves + central differences ---------------------
+    t_prep_b = 0.0
+    gpu_ms_b = 0.0
+    g_fd = np.zeros((n, 3))
+    sync()
+    t2 = time.perf_counter()
+    for i in range(n):
+        tgt = target[i]
+        msk = mask[i]
+        sc = scales[i]
+
+        def solve(p, m=None):
+            nonlocal t_prep_b, gpu_ms_b
+            if m is None:
+                tp = time.perf_counter()
+                m = rb.prepare_material(base, 3, *p, knot_threshold=KT)
+                t_prep_b += time.perf_counter() - tp
+            dm = solver.DeviceMaterial(m, 3, dev)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            E, _c, _k = solver.forward_histogram(geometry, source, m, O, int(Ns[i]), seed=seeds[i], device=dev,
+                                                 dmat=dm)
+            b.record()
+            de = E.to(torch.float64) * sc
+            r = torch.where(msk, torch.log(torch.where(msk, de, 1.0)) - torch.log(torch.where(msk, tgt, 1.0)), 0.0)
+            L = float((r * r).sum())
+            gpu_ms_b += a.elapsed_time(b)
+            return L
+
+        tp = time.perf_counter()
+        m0 = rb.prepare_material(base, 3, *params[i], knot_threshold=KT)
+        t_prep_b += time.perf_counter() - tp
+        solve(params[i], m0)
+        for k in range(3):
+            dp = np.zeros(3)
+            dp[k] = H[k] * params[i, k]
+            g_fd[i, k] = (solve(params[i] + dp) - solve(params[i] - dp)) / (2 * dp[k])
+    sync()
+    t_b = time.perf_counter() - t2
+
+    rel = np.abs(g_lr - g_fd) / np.maximum(np.abs(g_fd), 1e-300)
+    agree = {name: {"median_rel_diff": float(np.median(rel[:, k])), "within_2pct": int(np.sum(rel[:, k] < 0.02)),
+                    "within_10pct": int(np.sum(rel[:, k] < 0.10)),
+                    "sum_rel_diff": float(abs(g_lr[:, k].sum() - g_fd[:, k].sum()) / abs(g_fd[:, k].sum()))}
+             for k, name in enumerate(("c", "lc", "Q"))}
+    total_a = t_a + t_prep_a
+    total_b = t_b
+    out = {
+        "metric": "batched fwd+adjoint vs plain+FD wall time on 1 MI355X",
+        "value": total_b / total_a,
+        "unit": "x (plain+FD wall time / batched fwd+adjoint wall time, material preparation included)",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1000.0 * total_a,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Philox4x32-10 particles; 64 materials from numpy default_rng(20240604))",
+        "config": {"workload": "64 3-D acoustic materials, cv=[c c] c~U[.02,.15], lc~U[.05,.5], Q~U[50,1000], "
+                               "exp ACF, f=10, README geometry/observation; particles log-uniform 1e5..1e7 "
+                               "(config 4)", "particles_total": int(Ns.sum()), "batch_scale": args.batch_scale,
+                   "loss": "sum over bins with E_target>0 and >=16 counts of (log E - log E_target)^2; target at "
+                           "(+5% c, -5% lc, +10% Q), independent seed",
+                   "fd_steps_rel": H.tolist(), "tables": "knot_threshold=0"},
+        "batched": {"wall_s": total_a, "prep_s": t_prep_a, "solve_s": t_a, "forward_ms": ms_fwd,
+                    "adjoint_ms": ms_adj, "scatter_events": nscat, "deposits": ndep,
+                    "events_per_s_forward": nscat / (ms_fwd * 1e-3), "tuning": tuning},
+        "plain_fd": {"wall_s": total_b, "prep_s": t_prep_b, "solves": 7 * n, "gpu_ms": gpu_ms_b},
+        "gpu_only_speedup": gpu_ms_b / (ms_fwd + ms_adj),
+        "prep_excluded_speedup": (total_b - t_prep_b) / t_a,
+        "gradient_agreement": agree,
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -428,8 +428,11 @@ class MaterialClass:
         if np.any(np.isnan(pdf)):
             warnings.warn('There is a NaN inside the probability density function')
         cdf = np.nancumsum(pdf) * (np.pi / Nth)
-        ind = np.nonzero(np.diff(cdf) > knot_threshold)[0]
-        ind = np.unique(np.concatenate([ind, ind + 1]))
+        step = np.diff(cdf) > knot_threshold
+        keep = np.zeros(Nth, dtype=bool)          # i where step i or step i-1 is kept
+        keep[:-1] |= step
+        keep[1:] |= step
+        ind = np.nonzero(keep)[0]
         if knot_threshold <= 0:
             # cdf steps that underflow next to 1 would repeat a knot; beyond the
             # last knot (mass ~1/Nth) the table holds its last angle instead of
