@@ -184,6 +184,19 @@ typedef struct {
     int64_t mode_stride;        /* index step of mode 1 (S); unused for acoustics */
 } rtmc_bins;
 
+/* Histogram placement of the fused kernel (rtmc_problem.hist_mode).
+ * AUTO: LDS when the whole histogram (+ edges) fits in a work-group's LDS,
+ * otherwise TMAJOR (GLOBAL if its scratch cannot be allocated).
+ * TMAJOR deposits into a time-major scratch copy (time fastest), where a
+ * particle that stays in one bin over two consecutive observation times
+ * costs one 64-bit atomic instead of two, and adds it into E with a tiled
+ * transpose; it holds one scratch histogram per item in the device's default
+ * memory pool (kept by the pool for the next call). */
+#define RTMC_HIST_AUTO 0
+#define RTMC_HIST_GLOBAL 1
+#define RTMC_HIST_LDS 2
+#define RTMC_HIST_TMAJOR 3
+
 /* Whole forward problem for the fused kernel (radiativeTransfer.m:20-158). */
 typedef struct {
     rtmc_geometry geometry;
@@ -199,7 +212,8 @@ typedef struct {
     int32_t precision;          /* RTMC_PREC_* */
     int32_t block_size;         /* 0 = library default */
     int32_t particles_per_thread; /* 0 = library default */
-    int32_t hist_mode;          /* 0 = auto, 1 = global atomics, 2 = LDS-privatised */
+    int32_t hist_mode;          /* RTMC_HIST_*: 0 = auto, 1 = global atomics, 2 = LDS-privatised,
+                                   3 = time-major scratch with paired 64-bit atomics */
 } rtmc_problem;
 
 /* Information returned by rtmc_propagate_smalldt about its sub-steps. */
@@ -308,7 +322,7 @@ typedef struct {
     int32_t precision;          /* RTMC_PREC_* */
     int32_t block_size;         /* 0 = library default */
     int32_t particles_per_thread; /* 0 = library default (fill the device) */
-    int32_t hist_mode;          /* 0 = auto, 1 = global atomics, 2 = LDS-privatised */
+    int32_t hist_mode;          /* RTMC_HIST_* */
     int32_t event_batch;        /* 0 = default (24): lanes waiting before an event step */
     int32_t n_items;
     const rtmc_batch_item* items;   /* host array of n_items */

@@ -16,8 +16,33 @@ namespace rtmc {
 
 // defined in rtmc_trace_{2,3}d_{f64,f32}.hip
 template <int DIM, typename R>
-int launch_trace(const LaunchParams& p, const ProbDev* probs, bool ldsh, bool quant, bool ac, bool bnd, int sink,
+int launch_trace(const LaunchParams& p, const ProbDev* probs, int hm, bool quant, bool ac, bool bnd, int sink,
                  size_t smem, unsigned grid, unsigned block, hipStream_t st);
+
+// E[sp + it ts + m ms] += T[(sp + m ts) nt_pad + it]: the time-major scratch
+// added into the reference layout through 64 x 64 LDS tiles (coalesced reads
+// along time, coalesced writes along the bins; zero counts are skipped).
+__global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__ T, uint32_t* __restrict__ E,
+                                                    int64_t ts, int64_t ms, int nt, int nt_pad) {
+  __shared__ uint32_t tile[64][65];
+  const int64_t sp0 = (int64_t)blockIdx.x * 64;
+  const int it0 = blockIdx.y * 64;
+  const int64_t m = blockIdx.z;
+  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+    const int r = k >> 6, c = k & 63;
+    const int64_t sp = sp0 + r;
+    const int it = it0 + c;
+    tile[r][c] = (sp < ts && it < nt) ? T[(sp + m * ts) * nt_pad + it] : 0u;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+    const int c = k >> 6, r = k & 63;
+    const int64_t sp = sp0 + r;
+    const int it = it0 + c;
+    const uint32_t v = tile[r][c];
+    if (v && sp < ts && it < nt) E[sp + (int64_t)it * ts + m * ms] += v;
+  }
+}
 
 namespace {
 
@@ -25,10 +50,14 @@ namespace {
 struct Plan {
   LaunchParams p;
   int dim;
-  bool ldsh, quant, ac, bnd, fp32;
+  int hm;  // HM_*
+  bool quant, ac, bnd, fp32;
   size_t smem;
   unsigned grid, block;
   int64_t hsize;  // words of one problem's histogram
+  int n_modes;
+  int64_t tsize;  // HM_TMAJOR: words of one problem's time-major scratch
+  bool tmaj_auto; // HM_TMAJOR chosen by default (falls back to HM_GLOBAL without memory)
 };
 
 int check_shared(const rtmc_geometry& g, const rtmc_bins& B, int n_times, int time_steps, int precision) {
@@ -144,11 +173,18 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   RTMC_REQUIRE(guide >= 0 && guide <= (1 << 16) && (guide & (guide - 1)) == 0,
                "RTMC_GUIDE_CELLS must be 0 or a power of two <= 65536");
   auto guide_bytes = [&](int G) { return G > 0 ? (size_t)p.ntab * (size_t)(G + 1) * sizeof(int) : (size_t)0; };
+  RTMC_REQUIRE(hist_mode >= 0 && hist_mode <= 3, "hist_mode must be 0, 1, 2 or 3");
   bool ldsh;
-  if (hist_mode == 1) ldsh = false;
+  if (hist_mode == 1 || hist_mode == 3) ldsh = false;
   else if (hist_mode == 2) ldsh = true;
   else ldsh = (edge_bytes + hist_bytes) <= (size_t)max_smem && hsize <= (int64_t)1 << 20;
-  RTMC_REQUIRE(hist_mode >= 0 && hist_mode <= 2, "hist_mode must be 0, 1 or 2");
+  pl.n_modes = acoustics ? 1 : 2;
+  // time-major scratch: explicit (3), or by default whenever the histogram is in global memory
+  bool tmaj = !adjoint && !ldsh && n_times >= 2 && (hist_mode == 3 || hist_mode == 0);
+  p.nt_pad = n_times + (n_times & 1);
+  pl.tsize = tmaj ? B.time_stride * pl.n_modes * (int64_t)p.nt_pad : 0;
+  pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
+  pl.tmaj_auto = tmaj && hist_mode == 0;
   if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
     set_error("LDS %s of %zu bytes exceeds the %d bytes of LDS per work-group", adjoint ? "weight table" : "histogram",
               hist_bytes, max_smem);
@@ -181,7 +217,6 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   }
   RTMC_REQUIRE(p.batch >= 1 && p.batch <= 64, "event batch must be in [1, 64]");
   pl.dim = dim;
-  pl.ldsh = ldsh;
   pl.quant = time_steps == 0;
   pl.ac = acoustics != 0;
   pl.bnd = g.n_bnd > 0;
@@ -190,12 +225,45 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
 }
 
 // Copies the problems to the device, clears the counters, launches, frees.
-int run_plan(Plan& pl, const std::vector<ProbDev>& probs, int sink, hipStream_t st) {
+// The stream-ordered pool keeps what it has handed out (the time-major
+// scratch is GBs and is needed again by the next call).
+int keep_pool_memory() {
+  static thread_local int done_dev = -1;
+  int dev = 0;
+  RTMC_HIP(hipGetDevice(&dev));
+  if (done_dev == dev) return RTMC_OK;
+  hipMemPool_t pool;
+  RTMC_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t thr = UINT64_MAX;
+  RTMC_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  done_dev = dev;
+  return RTMC_OK;
+}
+
+int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   const int np = (int)probs.size();
   const size_t pbytes = sizeof(ProbDev) * (size_t)np;
   const size_t wbytes = sizeof(unsigned long long) * (size_t)np;
   const size_t abytes = sizeof(unsigned int) * (size_t)np;
   void* buf = nullptr;
+  void* tbuf = nullptr;
+  const size_t tbytes = sizeof(uint32_t) * (size_t)pl.tsize * (size_t)np;
+  if (pl.hm == HM_TMAJOR) {
+    int e = keep_pool_memory();
+    if (e) return e;
+    hipError_t ae = hipMallocAsync(&tbuf, tbytes, st);
+    if (ae != hipSuccess) {
+      tbuf = nullptr;
+      if (!pl.tmaj_auto) {
+        set_error("time-major scratch of %zu bytes: %s (use hist_mode 1)", tbytes, hipGetErrorString(ae));
+        return RTMC_ERR_NOMEM;
+      }
+      (void)hipGetLastError();
+      pl.hm = HM_GLOBAL;  // direct global atomics need no scratch
+    } else {
+      for (int i = 0; i < np; ++i) probs[i].T = (uint32_t*)tbuf + (size_t)pl.tsize * (size_t)i;
+    }
+  }
   RTMC_HIP(hipMallocAsync(&buf, pbytes + wbytes + abytes, st));
   char* b = (char*)buf;
   ProbDev* dprobs = (ProbDev*)b;
@@ -204,26 +272,41 @@ int run_plan(Plan& pl, const std::vector<ProbDev>& probs, int sink, hipStream_t 
   int rc = RTMC_OK;
   hipError_t e = hipMemcpyAsync(dprobs, probs.data(), pbytes, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemsetAsync(pl.p.work, 0, wbytes + abytes, st);
+  if (e == hipSuccess && tbuf) e = hipMemsetAsync(tbuf, 0, tbytes, st);
   if (e != hipSuccess) {
     set_error("problem upload failed: %s", hipGetErrorString(e));
     rc = RTMC_ERR_HIP;
   }
   if (rc == RTMC_OK) {
     if (pl.dim == 2) {
-      rc = pl.fp32 ? launch_trace<2, float>(pl.p, dprobs, pl.ldsh, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+      rc = pl.fp32 ? launch_trace<2, float>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                             pl.block, st)
-                   : launch_trace<2, double>(pl.p, dprobs, pl.ldsh, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+                   : launch_trace<2, double>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                              pl.block, st);
     } else {
-      rc = pl.fp32 ? launch_trace<3, float>(pl.p, dprobs, pl.ldsh, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+      rc = pl.fp32 ? launch_trace<3, float>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                             pl.block, st)
-                   : launch_trace<3, double>(pl.p, dprobs, pl.ldsh, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+                   : launch_trace<3, double>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                              pl.block, st);
     }
   }
+  if (rc == RTMC_OK && tbuf) {
+    const int64_t ts = pl.p.time_stride;
+    dim3 grid((unsigned)((ts + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
+    for (int i = 0; i < np && rc == RTMC_OK; ++i) {
+      hipLaunchKernelGGL(k_tmajor_add, grid, dim3(256), 0, st, (const uint32_t*)probs[i].T, probs[i].E, ts,
+                         pl.p.mode_stride, pl.p.n_times, pl.p.nt_pad);
+      hipError_t le = hipGetLastError();
+      if (le != hipSuccess) {
+        set_error("k_tmajor_add launch failed: %s", hipGetErrorString(le));
+        rc = RTMC_ERR_HIP;
+      }
+    }
+  }
   hipError_t fe = hipFreeAsync(buf, st);
-  if (rc == RTMC_OK && fe != hipSuccess) {
-    set_error("hipFreeAsync failed: %s", hipGetErrorString(fe));
+  hipError_t fe2 = tbuf ? hipFreeAsync(tbuf, st) : hipSuccess;
+  if (rc == RTMC_OK && (fe != hipSuccess || fe2 != hipSuccess)) {
+    set_error("hipFreeAsync failed: %s", hipGetErrorString(fe != hipSuccess ? fe : fe2));
     return RTMC_ERR_HIP;
   }
   return rc;
@@ -392,12 +475,13 @@ extern "C" int rtmc_autotune_batched(const rtmc_batch* batch, double sample_frac
     int block, ppt, hist, ev;
   };
   std::vector<Cand> cands;
-  const bool lds_possible = pl0.ldsh || batch->hist_mode == 2;
+  const bool lds_possible = pl0.hm == HM_LDS || batch->hist_mode == 2;
   std::vector<int> hists;
   if (batch->hist_mode != 0) hists.push_back(batch->hist_mode);
   else {
     if (lds_possible) hists.push_back(2);
     hists.push_back(1);
+    hists.push_back(3);
   }
   for (int h : hists) {
     std::vector<int> blocks = (h == 2) ? std::vector<int>{1024, 512} : std::vector<int>{256, 128};
@@ -435,7 +519,8 @@ extern "C" int rtmc_autotune_batched(const rtmc_batch* batch, double sample_frac
   rc = time_one(nullptr, &tdef);
   double tbest = tdef;
   // the library default, resolved
-  Cand cbest = {(int)pl0.block, batch->particles_per_thread, pl0.ldsh ? 2 : 1, pl0.p.batch};
+  Cand cbest = {(int)pl0.block, batch->particles_per_thread,
+                pl0.hm == HM_LDS ? 2 : (pl0.hm == HM_TMAJOR ? 3 : 1), pl0.p.batch};
   for (size_t i = 0; rc == RTMC_OK && i < cands.size(); ++i) {
     double t = 0.0;
     rc = time_one(&cands[i], &t);

@@ -30,9 +30,20 @@
 // left, and it then picks again.  With one problem this is a plain
 // persistent kernel with dynamic particle distribution.
 //
-// Histogram (forward): LDS-privatised per work-group when the whole
-// histogram of a problem fits in LDS (flushed with global atomics when the
-// work-group leaves the problem), otherwise global no-return atomics.
+// Histogram placement (forward), template parameter HM:
+//  HM_LDS    LDS-privatised per work-group when the whole histogram of a
+//            problem fits in LDS (flushed with global atomics when the
+//            work-group leaves the problem);
+//  HM_GLOBAL global no-return atomics into E;
+//  HM_TMAJOR global atomics into a time-major scratch copy T (time fastest,
+//            the time axis padded to even length) that is added into E by a
+//            tiled transpose afterwards.  A particle often stays in the same
+//            bin over consecutive observation times; a deposit at an even time
+//            waits one observation for its partner and the two go out as one
+//            64-bit atomic on the aligned pair of counters (no carry can cross:
+//            a counter never exceeds 2^32 - 1).  The histogram atomics are
+//            executed at the memory side at a fixed request rate, so fewer
+//            requests is what makes the kernel faster.
 //
 // Adjoint: the same kernel replays every trajectory from its Philox counter
 // (same draws, same events, same deposits) and, at each deposit into bin b
@@ -73,6 +84,7 @@ struct ProbDev {
   int useQ;
   double cost;        // estimated work per particle (scheduling weight)
   uint32_t* E;        // histogram (forward)
+  uint32_t* T;        // HM_TMAJOR scratch: T[(spatial + mode * time_stride) * nt_pad + it]
   unsigned long long* counters;  // RTMC_NCOUNTERS or null
   // adjoint
   const float* w;     // dL/dE per bin
@@ -86,6 +98,8 @@ struct ProbDev {
   const double* glc;  // d log sigma(theta) / d lc at theta_k = k pi / (n_glc - 1)
   int64_t n_glc;
 };
+
+enum { HM_GLOBAL = 0, HM_LDS = 1, HM_TMAJOR = 2 };
 
 struct LaunchParams {
   rtmc_geometry G;
@@ -104,6 +118,7 @@ struct LaunchParams {
   int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
   int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
   int64_t hist_size;  // words of the LDS histogram / weight table (0 = global memory)
+  int nt_pad;         // HM_TMAJOR: n_times rounded up to even (time stride of T is 1)
   int n_prob;
   unsigned long long* work;  // per-problem particle counters (zeroed)
   unsigned int* active;      // per-problem count of work-groups on it (zeroed)
@@ -337,14 +352,14 @@ __device__ __forceinline__ void birth(const rtmc_source& S, Rng& rng, R xo[3], R
   }
 }
 
-// Histogram index of an observation (observeTime.m:15-45 + histcounts), or -1.
+// Index of the observation's bin over the binned variables (observeTime.m:15-45
+// + histcounts), without the time and mode terms, or -1 when outside.
 template <typename R>
 __device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSetup& bs,
-                                             const double* __restrict__ sedges, int dim, const R X[3], R dz,
-                                             int isP, int it) {
+                                             const double* __restrict__ sedges, int dim, const R X[3], R dz) {
   double v[4];
   obs_vars(dim, p.G.frame, X, dz, bs.check, v);
-  int64_t idx = (int64_t)it * p.time_stride;
+  int64_t idx = 0;
   if (p.edges_total > 0) {
     // edges in LDS (`sedges` points into the work-group's LDS block)
 #pragma unroll
@@ -363,7 +378,6 @@ __device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSet
       idx += (int64_t)b * p.stride[k];
     }
   }
-  if (!isP) idx += p.mode_stride;
   return idx;
 }
 
@@ -403,9 +417,11 @@ __device__ __forceinline__ int pick_problem(const LaunchParams& p, const ProbDev
 // SINK: 0 forward histogram, 1 adjoint (weighted scores)
 // LDS-histogram kernels run one 1024-lane work-group per CU (the histogram
 // takes most of the LDS); the others use 256-lane work-groups.
-template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND, typename R, int SINK>
-__global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams p,
-                                                             const ProbDev* __restrict__ probs) {
+template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK>
+__global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const LaunchParams p,
+                                                                     const ProbDev* __restrict__ probs) {
+  constexpr bool LDSH = HM == HM_LDS;
+  constexpr bool TMAJ = HM == HM_TMAJOR;
   extern __shared__ double smem[];
   double* sedges = smem;
   __shared__ BinSetup s_bs;
@@ -485,8 +501,9 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
       }
       return table_eval(T.x, T.y, T.n, T.extrap, u);
     };
-    unsigned long long nscat = 0, ndep = 0, nbnd = 0, nkill = 0;
-    double ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;  // adjoint: dL/dc, dL/dlc, dL/dQ of this lane
+    uint32_t nscat = 0, ndep = 0, nbnd = 0, nkill = 0;  // this lane's, for this problem
+    // adjoint sums of this lane: w A, w (sum over events of dlogsigma - dlogSigma), w k
+    double gaA = 0.0, gaG = 0.0, gaK = 0.0;
     // ---- lane state machine ------------------------------------------------
     // Each lane holds one particle and its own next observation index.  One
     // iteration is either a DEPOSIT step (every flying lane deposits its next
@@ -508,8 +525,9 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
     int it = 1;                     // next observation index
     bool has = false;               // lane holds a particle
     bool need = true;               // lane wants a particle
+    int64_t tpend = -1;             // HM_TMAJOR: T index of a deposit waiting for its pair
     int nsc = 0;                    // adjoint: scatterings of the particle so far
-    double gsum = 0.0;              // adjoint: sum of dlogsigma_dlc(theta) over them
+    double gsum = 0.0;              // adjoint: sum of dlogsigma_dlc(theta) - dlogSigma_dlc over them
     // wave-uniform pool of particle indices fetched from the problem's counter
     int64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
@@ -640,7 +658,7 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
                   double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
                   if (ADJ) {
                     nsc += 1;
-                    gsum += glc_eval(P, th);
+                    gsum += glc_eval(P, th) - P.dlogS;
                   }
                   rotate_renorm(d, q, (R)th);
                   nscat += 1;
@@ -660,6 +678,10 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
             if (!alive) {
               has = false;
               need = true;
+              if (TMAJ && tpend >= 0) {
+                __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tpend = -1;
+              }
             }
           } else {
             // event-driven: scattering at time ts (propagateParticle.m:24-72)
@@ -672,7 +694,7 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
             double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
             if (ADJ) {
               nsc += 1;
-              gsum += glc_eval(P, th);
+              gsum += glc_eval(P, th) - P.dlogS;
             }
             rotate_renorm(d, q, (R)th);
             nscat += 1;
@@ -691,34 +713,48 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
             const R f = (R)((double)h * ((double)it * p.dt_obs - tb));
             for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
           }
-          const int64_t idx = bin_index(p, bs, sedges, DIM, X, d[2], isP, it);
-          if (idx >= 0) {
+          const int64_t sp = bin_index(p, bs, sedges, DIM, X, d[2]);
+          const int64_t idx = sp < 0 ? -1 : sp + (int64_t)it * p.time_stride + (isP ? 0 : p.mode_stride);
+          if (TMAJ && !ADJ) {
+            // time-major scratch: pair an even-time deposit with the next one
+            const int64_t j = sp < 0 ? -1 : (sp + (isP ? 0 : p.time_stride)) * (int64_t)p.nt_pad + it;
+            if (tpend >= 0 && j == tpend + 1) {
+              __hip_atomic_fetch_add((unsigned long long*)P.T + (tpend >> 1), 0x100000001ull, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+              tpend = -1;
+            } else {
+              if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              tpend = -1;
+              if (j >= 0) {
+                if (!(it & 1) && it + 1 < p.n_times) tpend = j;
+                else __hip_atomic_fetch_add(P.T + j, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+            }
+            if (j >= 0) ndep += 1;
+          } else if (idx >= 0) {
             ndep += 1;
             if (!ADJ) {
               if (LDSH) atomicAdd(shist + idx, 1u);
               else __hip_atomic_fetch_add(P.E + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-              // score of the trajectory up to this observation
-              double A, sq;
+              // score of the trajectory up to this observation: A = d log L / d log lam
+              // (scattering tests), k = removal tests survived
+              double A, k;
               if (QUANT) {
-                const double k = (double)it * (double)P.nsub;
-                const double lam = P.lamS[0];
-                A = (double)nsc * P.lam_over_expm1 - (k - (double)nsc) * lam;
-                sq = k * P.kq;
+                k = (double)it * (double)P.nsub;
+                A = (double)nsc * P.lam_over_expm1 - (k - (double)nsc) * P.lamS[0];
               } else {
+                k = 0.0;
                 A = (double)nsc - (double)it * p.dt_obs / M.mftP;
-                sq = 0.0;
               }
-              const double s0 = A * P.two_over_c;
-              const double s1 = A * P.dlogS + gsum - (double)nsc * P.dlogS;
               const double w = LDSH ? (double)sw[idx] : (double)P.w[idx];
-              ga0 += w * s0;
-              ga1 += w * s1;
-              ga2 += w * sq;
+              gaA += w * A;
+              gaG += w * gsum;
+              gaK += w * k;
               if (P.dhist) {
-                unsafeAtomicAdd(P.dhist + idx, s0);
-                unsafeAtomicAdd(P.dhist + P.dhist_stride + idx, s1);
-                if (sq != 0.0) unsafeAtomicAdd(P.dhist + 2 * P.dhist_stride + idx, sq);
+                unsafeAtomicAdd(P.dhist + idx, A * P.two_over_c);
+                unsafeAtomicAdd(P.dhist + P.dhist_stride + idx, A * P.dlogS + gsum);
+                if (P.kq != 0.0) unsafeAtomicAdd(P.dhist + 2 * P.dhist_stride + idx, k * P.kq);
               }
             }
           }
@@ -731,6 +767,7 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
       }
     }
 
+    if (TMAJ && tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (LDSH && !ADJ) {
       for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) {
@@ -739,30 +776,31 @@ __global__ __launch_bounds__(LDSH ? 1024 : 256) void k_trace(const LaunchParams 
       }
     }
     if (P.counters) {
-      // one atomic per wavefront and counter
+      // one atomic per wavefront and counter (64-bit sums of the lanes' 32-bit counts)
+      unsigned long long cs = nscat, cd = ndep, cb = nbnd, ck = nkill;
       for (int off = 32; off > 0; off >>= 1) {
-        nscat += __shfl_down(nscat, off, 64);
-        ndep += __shfl_down(ndep, off, 64);
-        nbnd += __shfl_down(nbnd, off, 64);
-        nkill += __shfl_down(nkill, off, 64);
+        cs += __shfl_down(cs, off, 64);
+        cd += __shfl_down(cd, off, 64);
+        cb += __shfl_down(cb, off, 64);
+        ck += __shfl_down(ck, off, 64);
       }
       if (lane == 0) {
-        if (nscat) atomicAdd(P.counters + RTMC_CNT_SCATTER, nscat);
-        if (ndep) atomicAdd(P.counters + RTMC_CNT_DEPOSIT, ndep);
-        if (nbnd) atomicAdd(P.counters + RTMC_CNT_BOUNDARY, nbnd);
-        if (nkill) atomicAdd(P.counters + RTMC_CNT_KILLED, nkill);
+        if (cs) atomicAdd(P.counters + RTMC_CNT_SCATTER, cs);
+        if (cd) atomicAdd(P.counters + RTMC_CNT_DEPOSIT, cd);
+        if (cb) atomicAdd(P.counters + RTMC_CNT_BOUNDARY, cb);
+        if (ck) atomicAdd(P.counters + RTMC_CNT_KILLED, ck);
       }
     }
     if (ADJ) {
       for (int off = 32; off > 0; off >>= 1) {
-        ga0 += __shfl_down(ga0, off, 64);
-        ga1 += __shfl_down(ga1, off, 64);
-        ga2 += __shfl_down(ga2, off, 64);
+        gaA += __shfl_down(gaA, off, 64);
+        gaG += __shfl_down(gaG, off, 64);
+        gaK += __shfl_down(gaK, off, 64);
       }
       if (lane == 0) {
-        unsafeAtomicAdd(P.grad + 0, ga0);
-        unsafeAtomicAdd(P.grad + 1, ga1);
-        unsafeAtomicAdd(P.grad + 2, ga2);
+        unsafeAtomicAdd(P.grad + 0, gaA * P.two_over_c);
+        unsafeAtomicAdd(P.grad + 1, gaA * P.dlogS + gaG);
+        unsafeAtomicAdd(P.grad + 2, gaK * P.kq);
       }
     }
     if (threadIdx.x == 0) atomicSub(p.active + cur, 1u);

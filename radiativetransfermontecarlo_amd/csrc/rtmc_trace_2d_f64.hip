@@ -2,6 +2,6 @@
 #include "rtmc_trace_launch.hpp"
 
 namespace rtmc {
-template int launch_trace<2, double>(const LaunchParams&, const ProbDev*, bool, bool, bool, bool, int, size_t,
+template int launch_trace<2, double>(const LaunchParams&, const ProbDev*, int, bool, bool, bool, int, size_t,
                                       unsigned, unsigned, hipStream_t);
 }  // namespace rtmc

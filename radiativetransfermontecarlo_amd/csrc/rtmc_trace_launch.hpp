@@ -8,10 +8,10 @@
 
 namespace rtmc {
 
-template <int DIM, bool LDSH, bool QUANT, bool AC, bool BND, typename R, int SINK>
+template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK>
 static int launch_one(const LaunchParams& p, const ProbDev* probs, size_t smem, unsigned grid, unsigned block,
                       hipStream_t st) {
-  auto kern = k_trace<DIM, LDSH, QUANT, AC, BND, R, SINK>;
+  auto kern = k_trace<DIM, HM, QUANT, AC, BND, R, SINK>;
   if (smem > 64 * 1024)
     RTMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), smem, st, p, probs);
@@ -19,39 +19,40 @@ static int launch_one(const LaunchParams& p, const ProbDev* probs, size_t smem, 
   return RTMC_OK;
 }
 
-template <int DIM, bool LDSH, typename R, int SINK>
+template <int DIM, int HM, typename R, int SINK>
 static int launch_proc(const LaunchParams& p, const ProbDev* probs, bool quant, bool ac, bool bnd, size_t smem,
                        unsigned grid, unsigned block, hipStream_t st) {
   if (SINK == 1) {
     if (quant) {
-      if (bnd) return launch_one<DIM, LDSH, true, true, true, R, 1>(p, probs, smem, grid, block, st);
-      return launch_one<DIM, LDSH, true, true, false, R, 1>(p, probs, smem, grid, block, st);
+      if (bnd) return launch_one<DIM, HM, true, true, true, R, 1>(p, probs, smem, grid, block, st);
+      return launch_one<DIM, HM, true, true, false, R, 1>(p, probs, smem, grid, block, st);
     }
-    return launch_one<DIM, LDSH, false, true, false, R, 1>(p, probs, smem, grid, block, st);
+    return launch_one<DIM, HM, false, true, false, R, 1>(p, probs, smem, grid, block, st);
   } else {
     if (quant) {
       if (ac) {
-        if (bnd) return launch_one<DIM, LDSH, true, true, true, R, 0>(p, probs, smem, grid, block, st);
-        return launch_one<DIM, LDSH, true, true, false, R, 0>(p, probs, smem, grid, block, st);
+        if (bnd) return launch_one<DIM, HM, true, true, true, R, 0>(p, probs, smem, grid, block, st);
+        return launch_one<DIM, HM, true, true, false, R, 0>(p, probs, smem, grid, block, st);
       }
-      if (bnd) return launch_one<DIM, LDSH, true, false, true, R, 0>(p, probs, smem, grid, block, st);
-      return launch_one<DIM, LDSH, true, false, false, R, 0>(p, probs, smem, grid, block, st);
+      if (bnd) return launch_one<DIM, HM, true, false, true, R, 0>(p, probs, smem, grid, block, st);
+      return launch_one<DIM, HM, true, false, false, R, 0>(p, probs, smem, grid, block, st);
     }
-    if (ac) return launch_one<DIM, LDSH, false, true, false, R, 0>(p, probs, smem, grid, block, st);
-    return launch_one<DIM, LDSH, false, false, false, R, 0>(p, probs, smem, grid, block, st);
+    if (ac) return launch_one<DIM, HM, false, true, false, R, 0>(p, probs, smem, grid, block, st);
+    return launch_one<DIM, HM, false, false, false, R, 0>(p, probs, smem, grid, block, st);
   }
 }
 
 template <int DIM, typename R>
-int launch_trace(const LaunchParams& p, const ProbDev* probs, bool ldsh, bool quant, bool ac, bool bnd, int sink,
+int launch_trace(const LaunchParams& p, const ProbDev* probs, int hm, bool quant, bool ac, bool bnd, int sink,
                  size_t smem, unsigned grid, unsigned block, hipStream_t st) {
   if (sink == 1) {
     RTMC_REQUIRE(ac, "the adjoint supports acoustic media only");
-    if (ldsh) return launch_proc<DIM, true, R, 1>(p, probs, quant, ac, bnd, smem, grid, block, st);
-    return launch_proc<DIM, false, R, 1>(p, probs, quant, ac, bnd, smem, grid, block, st);
+    if (hm == HM_LDS) return launch_proc<DIM, HM_LDS, R, 1>(p, probs, quant, ac, bnd, smem, grid, block, st);
+    return launch_proc<DIM, HM_GLOBAL, R, 1>(p, probs, quant, ac, bnd, smem, grid, block, st);
   }
-  if (ldsh) return launch_proc<DIM, true, R, 0>(p, probs, quant, ac, bnd, smem, grid, block, st);
-  return launch_proc<DIM, false, R, 0>(p, probs, quant, ac, bnd, smem, grid, block, st);
+  if (hm == HM_LDS) return launch_proc<DIM, HM_LDS, R, 0>(p, probs, quant, ac, bnd, smem, grid, block, st);
+  if (hm == HM_TMAJOR) return launch_proc<DIM, HM_TMAJOR, R, 0>(p, probs, quant, ac, bnd, smem, grid, block, st);
+  return launch_proc<DIM, HM_GLOBAL, R, 0>(p, probs, quant, ac, bnd, smem, grid, block, st);
 }
 
 }  // namespace rtmc

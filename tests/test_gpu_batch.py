@@ -250,3 +250,31 @@ def test_histograms_do_not_depend_on_launch_configuration(gpu):
     assert t["ms"] > 0 and t["ms"] <= t["ms_default"] * 1.0001
     assert t["block_size"] in (128, 256, 512, 1024) and t["hist_mode"] in (1, 2)
     assert torch.equal(B.forward()[0], E0)
+
+
+@pytest.mark.parametrize("tmax", [0.8, 0.75])   # 17 and 16 observation times (odd / even)
+def test_time_major_scratch_equals_direct_atomics(gpu, tmax):
+    """hist_mode 3 (time-major scratch, paired 64-bit atomics, transpose-add)
+    gives exactly the counts of direct global atomics and of the LDS histogram."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-3, 3, 17)   # 16 x 16 x 4 directions x 2 modes x 17 times fits in LDS
+    observation = {'x': e, 'y': e, 'directions': np.linspace(0, np.pi, 5),
+                   'time': np.arange(0, tmax + 1e-9, 0.05)}
+    m = cases.elastic_iso(2)
+    m.Q = [40.0, 20.0]
+    m.Frequency = 1.0
+    m = MaterialClass.prepareSigma(m, 2)
+    O = initializeObservation(geometry, False, observation, 1, allow_nd=True)
+    out = {}
+    for hm in (1, 2, 3):
+        E, cnt, _k = solver.forward_histogram(geometry, source, m, O, 300_001, seed=17, device='cuda', hist_mode=hm)
+        out[hm] = (E.clone(), cnt.clone())
+    assert int(out[1][0].sum()) > 0
+    for hm in (2, 3):
+        assert torch.equal(out[hm][0], out[1][0]), hm
+        assert torch.equal(out[hm][1], out[1][1]), hm
+    # accumulation semantics: a second call adds
+    E = out[3][0].clone()
+    solver.forward_histogram(geometry, source, m, O, 300_001, seed=17, device='cuda', hist_mode=3, E=E)
+    assert torch.equal(E, 2 * out[1][0])
