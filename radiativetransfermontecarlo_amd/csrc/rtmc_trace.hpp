@@ -352,18 +352,39 @@ __device__ __forceinline__ void birth(const rtmc_source& S, Rng& rng, R xo[3], R
   }
 }
 
+// Bin of the direction variable (observeTime.m:24-32: dir_z in 3-D, acos(dir_z)
+// in 2-D, as obs_vars computes it).  It changes only with the direction, so a
+// lane keeps it from one event to the next.  0 when the variable needs no
+// check, -1 when outside its edges.
+template <typename R>
+__device__ __forceinline__ int dir_bin(const LaunchParams& p, const BinSetup& bs, const double* __restrict__ sedges,
+                                       int dim, R dz) {
+  if (!(bs.check & 8u)) return 0;
+  R cp = dz;
+  if (cp > R(1)) cp = R(1);
+  if (cp < R(-1)) cp = R(-1);
+  const R psi = (dim == 3) ? cp : acos(cp);
+  const double* e = p.edges_total > 0 ? sedges + p.eoff[3] : p.edges[3];
+  return bin_guess(e, p.ne[3], bs.e0[3], bs.invw[3], (double)psi);
+}
+
 // Index of the observation's bin over the binned variables (observeTime.m:15-45
-// + histcounts), without the time and mode terms, or -1 when outside.
+// + histcounts), without the time and mode terms, or -1 when outside; the
+// direction's bin `dbin` comes from dir_bin.
 template <typename R>
 __device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSetup& bs,
-                                             const double* __restrict__ sedges, int dim, const R X[3], R dz) {
+                                             const double* __restrict__ sedges, int dim, const R X[3], int dbin) {
   double v[4];
-  obs_vars(dim, p.G.frame, X, dz, bs.check, v);
+  obs_vars(dim, p.G.frame, X, R(0), bs.check & 7u, v);
   int64_t idx = 0;
+  if (bs.check & 8u) {
+    if (dbin < 0) return -1;
+    idx = (int64_t)dbin * p.stride[3];
+  }
   if (p.edges_total > 0) {
     // edges in LDS (`sedges` points into the work-group's LDS block)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 3; ++k) {
       if (!(bs.check & (1u << k))) continue;
       int b = bin_guess(sedges + p.eoff[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
       if (b < 0) return -1;
@@ -371,7 +392,7 @@ __device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSet
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 3; ++k) {
       if (!(bs.check & (1u << k))) continue;
       int b = bin_guess(p.edges[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
       if (b < 0) return -1;
@@ -526,6 +547,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     bool has = false;               // lane holds a particle
     bool need = true;               // lane wants a particle
     int64_t tpend = -1;             // HM_TMAJOR: T index of a deposit waiting for its pair
+    int dbin = 0;                   // bin of the direction variable for the current direction
     int nsc = 0;                    // adjoint: scatterings of the particle so far
     double gsum = 0.0;              // adjoint: sum of dlogsigma_dlc(theta) - dlogSigma_dlc over them
     // wave-uniform pool of particle indices fetched from the problem's counter
@@ -578,6 +600,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             if (id >= 0) {
               rng.start(P.pid0 + (uint64_t)id);
               birth<DIM>(P.S, rng, xb, d, q);
+              dbin = dir_bin(p, bs, sedges, DIM, d[2]);
               isP = (!AC && P.S.polarization == 1) ? 0 : 1;
               it = 1;
               has = true;
@@ -673,6 +696,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                 xb[2] = x[2];
                 kb = j;
                 next_event();
+                dbin = dir_bin(p, bs, sedges, DIM, d[2]);
               }
             }
             if (!alive) {
@@ -697,6 +721,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               gsum += glc_eval(P, th) - P.dlogS;
             }
             rotate_renorm(d, q, (R)th);
+            dbin = dir_bin(p, bs, sedges, DIM, d[2]);
             nscat += 1;
             h = (R)(isP ? M.vp : M.vs);
             ts = tb - log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
@@ -713,7 +738,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             const R f = (R)((double)h * ((double)it * p.dt_obs - tb));
             for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
           }
-          const int64_t sp = bin_index(p, bs, sedges, DIM, X, d[2]);
+          const int64_t sp = bin_index(p, bs, sedges, DIM, X, dbin);
           const int64_t idx = sp < 0 ? -1 : sp + (int64_t)it * p.time_stride + (isP ? 0 : p.mode_stride);
           if (TMAJ && !ADJ) {
             // time-major scratch: pair an even-time deposit with the next one
