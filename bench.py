@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--ppt", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--atomic-probe", action="store_true",
+                    help="only measure the global-atomic request rate (rtmc_probe_atomics) over several ranges")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="flight arithmetic of the fused kernel (fp32: FP32 flight, FP64 time)")
     return ap.parse_args()
@@ -95,6 +97,8 @@ def main():
     import torch
     if args.config == 4:
         return run_batch_adjoint(args)
+    if args.atomic_probe:
+        return run_atomic_probe()
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.int32, device="cuda")
@@ -190,6 +194,25 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, geometry, source, material, O)
     print(json.dumps(out), flush=True)
+
+
+def run_atomic_probe():
+    """Request rate of no-return global atomics (the ceiling of global-memory deposits)."""
+    import ctypes as C
+    import torch
+    from radiativetransfermontecarlo_amd import _abi
+    lib = _abi.load()
+    words_max = 1_155_000_000  # the 4.6 GB histogram of the 2-D elastic workload
+    buf = torch.zeros(words_max, dtype=torch.int32, device="cuda")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rows = []
+    for kind, name in enumerate(("u32 random", "u64 random pairs", "u32 runs of 2", "u32 runs of 8")):
+        for words in (1 << 20, 1 << 25, 1 << 28, words_max):
+            r = C.c_double()
+            _abi.check(lib.rtmc_probe_atomics(C.c_void_p(buf.data_ptr()), words, kind, 2000, C.byref(r), stream),
+                       "rtmc_probe_atomics")
+            rows.append({"kind": name, "bytes": words * 4, "requests_per_s": r.value})
+    print(json.dumps({"atomic_probe": rows}), flush=True)
 
 
 def run_batch_adjoint(args):

@@ -375,6 +375,15 @@ typedef struct {
 int rtmc_autotune_batched(const rtmc_batch* batch, double sample_fraction,
                           rtmc_tuning* best, void* stream);
 
+/* Measurement aid, no counterpart in the reference: throughput of no-return
+ * global atomics (requests/s) on the current device, the ceiling of the
+ * fused kernel's deposits when the histogram is in global memory.  buf:
+ * device uint32[words] (contents are changed).  kind 0: uint32 at random
+ * words; 1: uint64 at random aligned pairs; 2 / 3: runs of 2 / 8 consecutive
+ * words from one lane.  Synchronises `stream`. */
+int rtmc_probe_atomics(uint32_t* buf, int64_t words, int32_t kind, int64_t iters,
+                       double* requests_per_s, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -251,6 +251,7 @@ EXPORTS = (
     "rtmc_forward_batched",
     "rtmc_adjoint_batched",
     "rtmc_autotune_batched",
+    "rtmc_probe_atomics",
 )
 
 _lib = None
@@ -284,6 +285,8 @@ def _declare(lib):
     lib.rtmc_adjoint_batched.argtypes = [P(Batch), P(AdjointItem), C.c_void_p]
     lib.rtmc_autotune_batched.restype = C.c_int
     lib.rtmc_autotune_batched.argtypes = [P(Batch), C.c_double, P(Tuning), C.c_void_p]
+    lib.rtmc_probe_atomics.restype = C.c_int
+    lib.rtmc_probe_atomics.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, P(C.c_double), C.c_void_p]
     return lib
 
 
