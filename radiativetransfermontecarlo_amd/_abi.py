@@ -252,6 +252,8 @@ EXPORTS = (
     "rtmc_adjoint_batched",
     "rtmc_autotune_batched",
     "rtmc_probe_atomics",
+    "rtmc_set_trace_timing",
+    "rtmc_trace_times",
 )
 
 _lib = None
@@ -285,6 +287,10 @@ def _declare(lib):
     lib.rtmc_adjoint_batched.argtypes = [P(Batch), P(AdjointItem), C.c_void_p]
     lib.rtmc_autotune_batched.restype = C.c_int
     lib.rtmc_autotune_batched.argtypes = [P(Batch), C.c_double, P(Tuning), C.c_void_p]
+    lib.rtmc_set_trace_timing.restype = C.c_int
+    lib.rtmc_set_trace_timing.argtypes = [C.c_int32]
+    lib.rtmc_trace_times.restype = C.c_int
+    lib.rtmc_trace_times.argtypes = [P(C.c_double), C.c_int32, P(C.c_int32)]
     lib.rtmc_probe_atomics.restype = C.c_int
     lib.rtmc_probe_atomics.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, P(C.c_double), C.c_void_p]
     return lib

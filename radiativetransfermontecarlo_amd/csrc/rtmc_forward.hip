@@ -225,6 +225,29 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
 }
 
 // Copies the problems to the device, clears the counters, launches, frees.
+// Optional timing of the tracing kernel launches (rtmc_set_trace_timing):
+// a per-thread ring of event pairs recorded on the launch stream.
+struct TraceTimer {
+  bool on = false;
+  int n = 0;
+  std::vector<hipEvent_t> ev;  // 2 per slot
+};
+thread_local TraceTimer g_timer;
+constexpr int kTimerSlots = 256;
+
+void timer_mark(hipStream_t st, int which) {
+  TraceTimer& T = g_timer;
+  if (!T.on || T.n >= kTimerSlots) return;
+  const size_t k = 2 * (size_t)T.n + (size_t)which;
+  while (T.ev.size() <= k) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    T.ev.push_back(e);
+  }
+  (void)hipEventRecord(T.ev[k], st);
+  if (which == 1) T.n += 1;
+}
+
 // The stream-ordered pool keeps what it has handed out (the time-major
 // scratch is GBs and is needed again by the next call).
 int keep_pool_memory() {
@@ -278,6 +301,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
     rc = RTMC_ERR_HIP;
   }
   if (rc == RTMC_OK) {
+    timer_mark(st, 0);
     if (pl.dim == 2) {
       rc = pl.fp32 ? launch_trace<2, float>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                             pl.block, st)
@@ -290,6 +314,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
                                              pl.block, st);
     }
   }
+  if (rc == RTMC_OK) timer_mark(st, 1);
   if (rc == RTMC_OK && tbuf) {
     const int64_t ts = pl.p.time_stride;
     dim3 grid((unsigned)((ts + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
@@ -548,5 +573,28 @@ extern "C" int rtmc_autotune_batched(const rtmc_batch* batch, double sample_frac
   best->event_batch = cbest.ev;
   best->ms = tbest;
   best->ms_default = tdef;
+  return RTMC_OK;
+}
+
+extern "C" int rtmc_set_trace_timing(int32_t on) {
+  clear_error();
+  g_timer.on = on != 0;
+  g_timer.n = 0;
+  return RTMC_OK;
+}
+
+extern "C" int rtmc_trace_times(double* ms, int32_t max, int32_t* n) {
+  clear_error();
+  RTMC_REQUIRE(n != nullptr && (ms != nullptr || max == 0), "ms and n must be given");
+  TraceTimer& T = g_timer;
+  int k = 0;
+  for (int i = 0; i < T.n && k < max; ++i, ++k) {
+    RTMC_HIP(hipEventSynchronize(T.ev[2 * i + 1]));
+    float t = 0.f;
+    RTMC_HIP(hipEventElapsedTime(&t, T.ev[2 * i], T.ev[2 * i + 1]));
+    ms[k] = t;
+  }
+  *n = k;
+  T.n = 0;
   return RTMC_OK;
 }
