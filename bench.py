@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="config 4: multiply every material's particle count (1 = the configuration's own)")
     ap.add_argument("--autotune", action="store_true", help="config 4: tune the batched launch first")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="config 2: skip the 64-material batched forward+adjoint comparison (config 4)")
     ap.add_argument("--particles", type=int, default=0, help="0 = the configuration's own count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -204,6 +206,16 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, geometry, source, material, O)
+    if args.config == 2 and not args.no_batch:
+        # the second half of the metric: batched fwd+adjoint vs plain+FD on the 64-material batch
+        del E
+        torch.cuda.empty_cache()
+        b = run_batch_adjoint(args, emit=False)
+        out["batched_fwd_adjoint_vs_plain_fd"] = {
+            "workload": b["config"]["workload"], "speedup_wall": b["value"],
+            "speedup_gpu_only": b["gpu_only_speedup"], "speedup_kernels_only": b["kernel_only_speedup"],
+            "batched": b["batched"], "plain_fd": b["plain_fd"],
+            "gradient_agreement": b["gradient_agreement"]}
     print(json.dumps(out), flush=True)
 
 
@@ -226,7 +238,7 @@ def run_atomic_probe():
     print(json.dumps({"atomic_probe": rows}), flush=True)
 
 
-def run_batch_adjoint(args):
+def run_batch_adjoint(args, emit=True):
     """64 acoustic materials: batched forward + adjoint against 64 plain forward
     solves with central finite differences (7 solves per material)."""
     import numpy as np
@@ -243,7 +255,7 @@ def run_batch_adjoint(args):
     O = initializeObservation(geometry, True, observation, 1)
     KT = 0.0  # tables that follow sigma: the score differentiates sigma
     H = np.array([0.02, 0.02, 0.05])  # central-difference steps (relative)
-    scales = torch.stack([torch.as_tensor(rb.count_scale(O, int(N)), device=dev) for N in Ns])
+    scales = rb.count_scales(O, [int(N) for N in Ns], dev)
 
     def sync():
         torch.cuda.synchronize()
@@ -254,6 +266,17 @@ def run_batch_adjoint(args):
                   list(Ns), [7_000 + i for i in range(n)], device=dev)
     Bt.forward()
     target = Bt.energy_density()
+
+    def loss_and_weights(E):
+        """loss per item and dL/dcount (the backward of the loss through the normalisation)"""
+        dens = E.to(torch.float64) * scales
+        mask = (target > 0) & (E >= 16)
+        lg = torch.where(mask, torch.log(torch.where(mask, dens, 1.0)) - torch.log(torch.where(mask, target, 1.0)),
+                         0.0)
+        return (lg * lg).sum(dim=1), torch.where(mask, 2.0 * lg / torch.where(mask, dens, 1.0), 0.0) * scales, mask
+
+    loss_and_weights(Bt.E)  # loads the elementwise kernels outside the timed regions
+    sync()
     del Bt
     seeds = [1_000 + i for i in range(n)]
 
@@ -264,24 +287,29 @@ def run_batch_adjoint(args):
     B = rb.Batch(geometry, O, mats, source, list(Ns), seeds, device=dev)
     tabs = B.derivative_tables()
     t_prep_a = time.perf_counter() - t0
-    tuning = B.autotune(0.02) if args.autotune else None
+    tuning = B.autotune(0.02) if (args.autotune or not emit) else None
+    import ctypes as C
+    from radiativetransfermontecarlo_amd import _abi
+    lib = _abi.load()
     sync()
+    _abi.check(lib.rtmc_set_trace_timing(1), "rtmc_set_trace_timing")
     t1 = time.perf_counter()
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     e0.record()
     B.forward()
     e1.record()
-    dens = B.E.to(torch.float64) * scales
-    mask = (target > 0) & (B.E >= 16)
-    lg = torch.where(mask, torch.log(torch.where(mask, dens, 1.0)) - torch.log(torch.where(mask, target, 1.0)), 0.0)
-    loss = (lg * lg).sum(dim=1)
-    w = torch.where(mask, 2.0 * lg / torch.where(mask, dens, 1.0), 0.0) * scales
+    loss, w, mask = loss_and_weights(B.E)
     grad = B.adjoint(w, tables=tabs)
     e2.record()
     sync()
     t_a = time.perf_counter() - t1
     g_lr = grad.cpu().numpy()
     ms_fwd, ms_adj = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    tk = (C.c_double * 4)()
+    nk = C.c_int32()
+    _abi.check(lib.rtmc_trace_times(tk, 4, C.byref(nk)), "rtmc_trace_times")
+    _abi.check(lib.rtmc_set_trace_timing(0), "rtmc_set_trace_timing")
+    k_fwd, k_adj = (tk[0], tk[1]) if nk.value >= 2 else (float("nan"), float("nan"))
     nscat = int(B.counters[:, 0].sum())
     ndep = int(B.counters[:, 1].sum())
     mask_np = mask.cpu().numpy()
@@ -289,6 +317,8 @@ def run_batch_adjoint(args):
     # ---- plain forward solves + central differences ---------------------
     t_prep_b = 0.0
     gpu_ms_b = 0.0
+    dbins = solver.DeviceBins(O, dev)
+    _abi.check(lib.rtmc_set_trace_timing(1), "rtmc_set_trace_timing")
     g_fd = np.zeros((n, 3))
     sync()
     t2 = time.perf_counter()
@@ -307,7 +337,7 @@ def run_batch_adjoint(args):
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
             E, _c, _k = solver.forward_histogram(geometry, source, m, O, int(Ns[i]), seed=seeds[i], device=dev,
-                                                 dmat=dm)
+                                                 dmat=dm, bins=dbins)
             b.record()
             de = E.to(torch.float64) * sc
             r = torch.where(msk, torch.log(torch.where(msk, de, 1.0)) - torch.log(torch.where(msk, tgt, 1.0)), 0.0)
@@ -325,6 +355,10 @@ def run_batch_adjoint(args):
             g_fd[i, k] = (solve(params[i] + dp) - solve(params[i] - dp)) / (2 * dp[k])
     sync()
     t_b = time.perf_counter() - t2
+    tkb = (C.c_double * 1024)()
+    _abi.check(lib.rtmc_trace_times(tkb, 1024, C.byref(nk)), "rtmc_trace_times")
+    _abi.check(lib.rtmc_set_trace_timing(0), "rtmc_set_trace_timing")
+    kernel_ms_b = float(sum(tkb[i] for i in range(nk.value)))
 
     rel = np.abs(g_lr - g_fd) / np.maximum(np.abs(g_fd), 1e-300)
     agree = {name: {"median_rel_diff": float(np.median(rel[:, k])), "within_2pct": int(np.sum(rel[:, k] < 0.02)),
@@ -348,14 +382,19 @@ def run_batch_adjoint(args):
                            "(+5% c, -5% lc, +10% Q), independent seed",
                    "fd_steps_rel": H.tolist(), "tables": "knot_threshold=0"},
         "batched": {"wall_s": total_a, "prep_s": t_prep_a, "solve_s": t_a, "forward_ms": ms_fwd,
-                    "adjoint_ms": ms_adj, "scatter_events": nscat, "deposits": ndep,
+                    "adjoint_ms": ms_adj, "forward_kernel_ms": k_fwd, "adjoint_kernel_ms": k_adj,
+                    "scatter_events": nscat, "deposits": ndep,
                     "events_per_s_forward": nscat / (ms_fwd * 1e-3), "tuning": tuning},
-        "plain_fd": {"wall_s": total_b, "prep_s": t_prep_b, "solves": 7 * n, "gpu_ms": gpu_ms_b},
+        "plain_fd": {"wall_s": total_b, "prep_s": t_prep_b, "solves": 7 * n, "gpu_ms": gpu_ms_b,
+                     "kernel_ms": kernel_ms_b},
         "gpu_only_speedup": gpu_ms_b / (ms_fwd + ms_adj),
+        "kernel_only_speedup": kernel_ms_b / (k_fwd + k_adj),
         "prep_excluded_speedup": (total_b - t_prep_b) / t_a,
         "gradient_agreement": agree,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -386,7 +386,7 @@ int rtmc_probe_atomics(uint32_t* buf, int64_t words, int32_t kind, int64_t iters
 
 /* Measurement aid, no counterpart in the reference: when on, every forward
  * or adjoint call on this thread records HIP events around its tracing
- * kernel on the call's stream (at most 256 launches are kept).
+ * kernel on the call's stream (at most 1024 launches are kept).
  * rtmc_trace_times waits for them and returns their durations in ms (at
  * most `max`, count in *n), then forgets them. */
 int rtmc_set_trace_timing(int32_t on);

@@ -25,7 +25,7 @@ from .material import MaterialClass, _integral
 from .observation import Observation, normalize
 from .solver import (DeviceBins, DeviceMaterial, _device, _f64, _get, _stream_ptr, build_geometry, build_source)
 
-__all__ = ["prepare_material", "lc_derivatives", "Batch", "EnergyDensity", "count_scale"]
+__all__ = ["prepare_material", "lc_derivatives", "Batch", "EnergyDensity", "count_scale", "count_scales"]
 
 
 def prepare_material(base: MaterialClass, d: int, c: Optional[float] = None, lc: Optional[float] = None,
@@ -91,6 +91,13 @@ def count_scale(O: Observation, N: int) -> np.ndarray:
     ones = np.ones(O.size, dtype=np.float64)
     dens, _ = normalize(O, ones, N)
     return np.asarray(dens, dtype=np.float64).reshape(-1, order='F')
+
+
+def count_scales(O: Observation, Ns: Sequence[int], device) -> torch.Tensor:
+    """count_scale of every item (n x O.size) with one host-to-device copy."""
+    base = torch.as_tensor(count_scale(O, 1), device=device)
+    inv = torch.as_tensor([1.0 / float(N) for N in Ns], dtype=torch.float64, device=device)
+    return inv[:, None] * base[None, :]
 
 
 class Batch:
@@ -192,28 +199,30 @@ class Batch:
         grad = torch.zeros((self.n, 3), dtype=torch.float64, device=self.dev)
         dh = torch.zeros((self.n, 3, self.O.size), dtype=torch.float64, device=self.dev) if dhist else None
         adj = (_abi.AdjointItem * max(self.n, 1))()
-        keep = []
+        # one upload for all derivative tables
+        ng = max(int(np.size(t[2])) for t in tables)
+        host = np.zeros((self.n, ng))
+        for i, (_c, _dS, g) in enumerate(tables):
+            host[i, :np.size(g)] = g
+        gt = _f64(host, self.dev)
         for i, (c, dS, g) in enumerate(tables):
-            gt = _f64(g, self.dev)
-            keep.append(gt)
             a = adj[i]
             a.weight = w[i].data_ptr()
             a.grad = grad[i].data_ptr()
             a.dhist = dh[i].data_ptr() if dh is not None else None
             a.cv = float(c)
             a.dlogSigma_dlc = float(dS)
-            a.dlogsigma_dlc = gt.data_ptr()
-            a.n_dlogsigma = int(g.size)
+            a.dlogsigma_dlc = gt[i].data_ptr()
+            a.n_dlogsigma = int(np.size(g))
         lib = _abi.load()
         _abi.check(lib.rtmc_adjoint_batched(C.byref(self.c), adj, C.c_void_p(_stream_ptr(self.dev))),
                    "rtmc_adjoint_batched")
-        torch.cuda.current_stream(self.dev).synchronize()  # keeps `keep` alive until the kernel is done
+        torch.cuda.current_stream(self.dev).synchronize()  # keeps `gt` alive until the kernel is done
         return (grad, dh) if dhist else grad
 
     def energy_density(self) -> torch.Tensor:
         """energyDensity of every item, flat (n x O.size, float64)."""
-        s = torch.stack([torch.as_tensor(count_scale(self.O, N), device=self.dev) for N in self.Ns])
-        return self.E.to(torch.float64) * s
+        return self.E.to(torch.float64) * count_scales(self.O, self.Ns, self.dev)
 
 
 class EnergyDensity(torch.autograd.Function):
@@ -241,7 +250,6 @@ class EnergyDensity(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
         B = ctx.B
-        s = torch.stack([torch.as_tensor(count_scale(B.O, N), device=B.dev) for N in B.Ns])
-        w = (grad_out.to(B.dev, torch.float64) * s)
+        w = grad_out.to(B.dev, torch.float64) * count_scales(B.O, B.Ns, B.dev)
         g = B.adjoint(w)
         return g.to(torch.float64).cpu(), None

@@ -233,7 +233,7 @@ struct TraceTimer {
   std::vector<hipEvent_t> ev;  // 2 per slot
 };
 thread_local TraceTimer g_timer;
-constexpr int kTimerSlots = 256;
+constexpr int kTimerSlots = 1024;
 
 void timer_mark(hipStream_t st, int which) {
   TraceTimer& T = g_timer;

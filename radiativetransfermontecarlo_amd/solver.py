@@ -401,7 +401,7 @@ def forward_histogram(geometry, source, material: MaterialClass, O: Observation,
                       device, time_steps: int = 0, particle_offset: int = 0, E: Optional[torch.Tensor] = None,
                       counters: Optional[torch.Tensor] = None, block_size: int = 0, hist_mode: int = 0,
                       particles_per_thread: int = 0, dmat: Optional[DeviceMaterial] = None,
-                      precision: str = "fp64"):
+                      precision: str = "fp64", bins: Optional[DeviceBins] = None):
     """One rtmc_forward call: adds the counts of N particles into E (flat int32
     device tensor, allocated when None).  Returns (E, counters, keep-alive)."""
     d = int(_get(geometry, 'dimension'))
@@ -413,7 +413,8 @@ def forward_histogram(geometry, source, material: MaterialClass, O: Observation,
     if dmat is None:
         dmat = DeviceMaterial(material, d, dev, with_zoeppritz=prob.geometry.n_bnd > 0)
     prob.material = dmat.c
-    bins = DeviceBins(O, dev)
+    if bins is None:
+        bins = DeviceBins(O, dev)
     prob.bins = bins.c
     prob.n_particles = int(N)
     prob.particle_offset = int(particle_offset)

@@ -278,3 +278,21 @@ def test_time_major_scratch_equals_direct_atomics(gpu, tmax):
     E = out[3][0].clone()
     solver.forward_histogram(geometry, source, m, O, 300_001, seed=17, device='cuda', hist_mode=3, E=E)
     assert torch.equal(E, 2 * out[1][0])
+
+
+def test_energy_density_autograd_function(gpu):
+    """EnergyDensity.apply / backward equal Batch.forward / Batch.adjoint."""
+    geometry, source, base, O, _m = _acoustic(tmax=0.6)
+    Ns, seeds = [100_000, 50_000], [3, 4]
+    params = torch.tensor([[0.08, 0.2, 300.0], [0.05, 0.3, 100.0]], dtype=torch.float64, requires_grad=True)
+    spec = dict(base=base, geometry=geometry, O=O, sources=source, Ns=Ns, seeds=seeds, device='cuda')
+    dens = rb.EnergyDensity.apply(params, spec)
+    R = torch.rand(dens.shape, generator=torch.Generator().manual_seed(5), dtype=torch.float64).to(dens.device)
+    (dens * R).sum().backward()
+    mats = [rb.prepare_material(base, 3, *p, knot_threshold=0.0) for p in params.detach().numpy()]
+    B = rb.Batch(geometry, O, mats, source, Ns, seeds, device='cuda')
+    B.forward()
+    assert torch.equal(dens.detach(), B.energy_density())
+    g = B.adjoint(R * rb.count_scales(O, Ns, 'cuda')).cpu()
+    assert params.grad.shape == (2, 3)
+    assert torch.allclose(params.grad, g, rtol=1e-9, atol=0)
