@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, choices=[1, 2, 4], default=2)
+    ap.add_argument("--config", type=int, choices=[1, 2, 3, 4, 5], default=2)
+    ap.add_argument("--chunks", type=int, default=0,
+                    help="forward calls per step, each on its own particle range (0 = the configuration's own)")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="config 4: multiply every material's particle count (1 = the configuration's own)")
     ap.add_argument("--autotune", action="store_true", help="config 4: tune the batched launch first")
@@ -115,21 +117,39 @@ def main():
     from tests import cases
 
     dev = torch.device("cuda", 0)
+    chunks = 1
     if args.config == 2:
         N = args.particles or 10_000_000
         geometry, source, material, observation = cases.config2(n=N)
         workload = ("2-D elastic P-SV, cartesian, 1e7 particles, Q=500, x,y=-50:0.5:50, 36 direction bins, "
                     "t=0:0.05:20 (config 2)")
+    elif args.config == 3:
+        N = args.particles or 50_000_000
+        geometry, source, material, observation = cases.config3(n=N)
+        workload = ("3-D elastic half-space with free surface at z=0, cylindrical, 5e7 particles, von Karman "
+                    "nu=0.5 lc=0.2 f=20, r=0:0.1:30, 90 direction bins, t=0:0.05:15 (config 3)")
+    elif args.config == 5:
+        # one launch streams all 1e9 particles (no particle state in HBM); --chunks 10 makes ten
+        # calls of 1e8 particles each, as the configuration is described
+        N = args.particles or 1_000_000_000
+        geometry, source, material, observation = cases.config5(n=N)
+        workload = ("3-D acoustic, cartesian, 1e9 particles, exp ACF lc=0.05 f=40 cv=0.1 Q=300, 121^3 cells of "
+                    "0.1, 8 direction bins, t=0:0.02:10 (config 5)")
     else:
         N = args.particles or 1_000_000
         geometry, source, material, observation = cases.config1(n=N)
         workload = ("3-D acoustic, spherical, 1e6 particles, exp ACF lc=0.1, f=10, x=0:0.05:6, t=0:0.02:5 "
                     "(config 1, the reference's README case)")
+    chunks = args.chunks or chunks
+    if chunks > 1:
+        workload += f", {chunks} calls of {N // chunks} particles"
     args.particles = N
     d = geometry['dimension']
     material = MaterialClass.prepareSigma(material, d)
     O = initializeObservation(geometry, bool(material.acoustics), observation, N, allow_nd=True)
-    dmat = solver.DeviceMaterial(material, d, dev)
+    dmat = solver.DeviceMaterial(material, d, dev, with_zoeppritz=len(geometry.get('bnd', [])) > 0)
+    dbins = solver.DeviceBins(O, dev)
+    bounds = [N * k // chunks for k in range(chunks + 1)]
     E = torch.zeros(O.size, dtype=torch.int32, device=dev)
     cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -140,9 +160,11 @@ def main():
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
-        keep = solver.forward_histogram(geometry, source, material, O, N, seed=seed, device=dev, E=E, counters=cnt,
-                                        dmat=dmat, block_size=args.block, particles_per_thread=args.ppt,
-                                        precision=args.precision)
+        for k in range(chunks):
+            keep = solver.forward_histogram(geometry, source, material, O, bounds[k + 1] - bounds[k], seed=seed,
+                                            device=dev, E=E, counters=cnt, dmat=dmat, block_size=args.block,
+                                            particles_per_thread=args.ppt, precision=args.precision,
+                                            particle_offset=bounds[k], bins=dbins)
         ev1.record(stream)
         return ev0, ev1, keep
 
@@ -150,7 +172,8 @@ def main():
     lib = _abi.load()
     for w in range(args.warmup):
         step(1000 + w)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        print(f"warmup step {w} done", file=sys.stderr, flush=True)
     call_ms = []
     events = []
     _abi.check(lib.rtmc_set_trace_timing(1), "rtmc_set_trace_timing")
@@ -158,6 +181,9 @@ def main():
     for s in range(args.steps):
         ev0, ev1, keep = step(s)
         events.append((ev0, ev1))
+        if args.config == 5:  # long steps: a progress line each
+            torch.cuda.synchronize()
+            print(f"step {s} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     for ev0, ev1 in events:
@@ -172,12 +198,12 @@ def main():
     nscat = int(c[_abi.CNT_SCATTER])
     ndep = int(c[_abi.CNT_DEPOSIT])
     ms = 1000.0 * wall / args.steps
-    kavg = float(np.mean(kern_ms))
+    kavg = float(np.sum(kern_ms)) / max(1, args.steps)  # tracing-kernel time per step (all chunks)
     cavg = float(np.mean(call_ms))
-    if args.config == 2:
+    if args.config in (2, 3, 5):
         achieved = BYTES_PER_DEPOSIT * ndep / (kavg * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles() if args.config == 2 else None,
                 "kernel": "k_trace", "kernel_ms": kavg, "forward_call_ms": cavg,
                 "algorithmic_unit": "8 B per histogram deposit (4 B read + 4 B write of the counter)"}
     else:

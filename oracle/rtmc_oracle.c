@@ -37,6 +37,8 @@
 
 /* scattering events of the calling thread (read by orc_forward) */
 static _Thread_local uint64_t tl_nscat = 0;
+/* set by orc_forward when the threads share one histogram (too large to copy per thread) */
+static _Thread_local int tl_shared_hist = 0;
 
 /* ------------------------------------------------------------------ */
 /* Philox4x32-10 (SC'11 paper, Sec. 2 and Table 2)                     */
@@ -701,7 +703,8 @@ int orc_observe(const rtmc_geometry* G, const rtmc_bins* B, int32_t acoustics,
         }
         if (!ok) continue;
         if (!acoustics && !P->p[i]) idx += B->mode_stride;                    /* :144-146 */
-        E[idx] += 1u;
+        if (tl_shared_hist) __atomic_fetch_add(&E[idx], 1u, __ATOMIC_RELAXED);
+        else E[idx] += 1u;
     }
     return RTMC_OK;
 }
@@ -728,12 +731,20 @@ int orc_forward(const rtmc_problem* prob, int64_t packet, uint32_t* E, int32_t n
     int err = RTMC_OK;
     uint64_t tot_alive_deposits = 0, tot_scat = 0;
     (void)nmode;
+    /* thread-private histograms (summed at the end, like parfor's reduction of
+       E_local, radiativeTransfer.m:65-88) unless they would take over 256 MiB
+       each: then every thread adds into E with atomic increments */
+    const int shared = (size_t)hsize * sizeof(uint32_t) > ((size_t)256 << 20);
+    uint64_t dep_shared0 = 0;
+    if (shared)
+        for (int64_t k = 0; k < hsize; ++k) dep_shared0 += E[k];
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #pragma omp parallel
 #endif
     {
-        uint32_t* El = (uint32_t*)calloc((size_t)hsize, sizeof(uint32_t));
+        tl_shared_hist = shared;
+        uint32_t* El = shared ? E : (uint32_t*)calloc((size_t)hsize, sizeof(uint32_t));
         double* buf = (double*)malloc(sizeof(double) * (size_t)(10 * packet + 16));
         uint8_t* bb = (uint8_t*)malloc((size_t)(2 * packet + 16));
         uint64_t dep = 0;
@@ -767,11 +778,20 @@ int orc_forward(const rtmc_problem* prob, int64_t packet, uint32_t* E, int32_t n
 #pragma omp critical
 #endif
         {
-            for (int64_t k = 0; k < hsize; ++k) { E[k] += El[k]; dep += El[k]; }
+            if (!shared) {
+                for (int64_t k = 0; k < hsize; ++k) { E[k] += El[k]; dep += El[k]; }
+                free(El);
+            }
             tot_alive_deposits += dep;
             tot_scat += tl_nscat;
         }
-        free(El); free(buf); free(bb);
+        tl_shared_hist = 0;
+        free(buf); free(bb);
+    }
+    if (shared) {
+        uint64_t s = 0;
+        for (int64_t k = 0; k < hsize; ++k) s += E[k];
+        tot_alive_deposits = s - dep_shared0;
     }
     if (counters) {
         counters[RTMC_CNT_SCATTER] = tot_scat;

@@ -88,3 +88,18 @@ def config4(n_items=64, seed=20240604):
     N = np.round(10.0 ** rng.uniform(5.0, 7.0, n_items)).astype(np.int64)
     params = np.stack([c, lc, Q], axis=1)
     return geometry, source, base, observation, params, N
+
+
+def config5(n=1_000_000_000):
+    """Large 3-D acoustic, cartesian frame: exp ACF lc = 0.05, f = 40, cv = [0.1 0.1],
+    Q = 300; 121^3 cells of 0.1 (edges -6.05:0.1:6.05), 8 direction bins,
+    t = 0:0.02:10 (four binned variables, a 28.4 GB histogram)."""
+    geometry = {'dimension': 3, 'frame': 'cartesian'}
+    source = {'numberParticles': n, 'type': 'point', 'position': [0, 0, 0], 'direction': 'uniform',
+              'lambda': 0.05}
+    edges = np.round(np.arange(-6.05, 6.05 + 1e-9, 0.1), 10)
+    observation = {'x': edges, 'y': edges, 'z': edges, 'directions': np.linspace(0, np.pi, 9),
+                   'time': np.arange(0, 10 + 1e-9, 0.02)}
+    material = MaterialClass(geometry, 40, True, 1, [0.1, 0.1], 0.0, 'exp', 0.05)
+    material.Q = 300.0
+    return geometry, source, material, observation
