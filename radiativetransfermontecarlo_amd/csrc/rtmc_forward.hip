@@ -193,9 +193,27 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // the guide takes what the histogram leaves
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
+  // time-major mode: a cache of hot counters, 12 B a slot; the guide shrinks so that the
+  // work-group stays within 40 KB of LDS (four 256-lane work-groups per CU)
+  int hot = 0;
+  if (tmaj) {
+    hot = 1024;
+    if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
+    RTMC_REQUIRE(hot >= 0 && hot <= (1 << 14) && (hot & (hot - 1)) == 0,
+                 "RTMC_HOT_SLOTS must be 0 or a power of two <= 16384");
+    const size_t budget = 40 * 1024;
+    while (guide > 256 && edge_bytes + guide_bytes(guide) + 8 + 12 * (size_t)hot > budget) guide >>= 1;
+  }
   p.guide_n = guide;
   p.hist_size = ldsh ? hsize : 0;
   pl.smem = edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0);
+  p.hot_slots = hot;
+  p.hot_times = 1 << 30;
+  if (const char* h = getenv("RTMC_HOT_TIMES")) p.hot_times = atoi(h);
+  p.hot_shift = 64;
+  for (int v = hot; v > 1; v >>= 1) p.hot_shift -= 1;
+  p.hot_off = (int)((pl.smem + 7) & ~(size_t)7);
+  if (hot) pl.smem = (size_t)p.hot_off + 12 * (size_t)hot;
 
   unsigned block = block_size > 0 ? (unsigned)block_size : (ldsh ? 1024u : 256u);
   RTMC_REQUIRE(block % 64 == 0 && block <= (ldsh ? 1024u : 256u),
@@ -283,6 +301,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       }
       (void)hipGetLastError();
       pl.hm = HM_GLOBAL;  // direct global atomics need no scratch
+      pl.p.hot_slots = 0;
     } else {
       for (int i = 0; i < np; ++i) probs[i].T = (uint32_t*)tbuf + (size_t)pl.tsize * (size_t)i;
     }

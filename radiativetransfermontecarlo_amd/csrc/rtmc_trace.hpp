@@ -43,7 +43,13 @@
 //            64-bit atomic on the aligned pair of counters (no carry can cross:
 //            a counter never exceeds 2^32 - 1).  The histogram atomics are
 //            executed at the memory side at a fixed request rate, so fewer
-//            requests is what makes the kernel faster.
+//            requests is what makes the kernel faster.  Counters hit by many
+//            particles (near the source at early times every particle lands in
+//            the same few bins, and same-address atomics serialise) are
+//            aggregated in a small per-work-group LDS cache first: a
+//            direct-mapped table of (T index, count) slots, each claimed by the
+//            first index that maps to it and flushed when the work-group leaves
+//            the problem.
 //
 // Adjoint: the same kernel replays every trajectory from its Philox counter
 // (same draws, same events, same deposits) and, at each deposit into bin b
@@ -119,6 +125,10 @@ struct LaunchParams {
   int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
   int64_t hist_size;  // words of the LDS histogram / weight table (0 = global memory)
   int nt_pad;         // HM_TMAJOR: n_times rounded up to even (time stride of T is 1)
+  int hot_slots;      // HM_TMAJOR: slots of the LDS cache of hot counters (power of two, or 0)
+  int hot_shift;      // 64 - log2(hot_slots)
+  int hot_off;        // byte offset of the cache in the dynamic LDS block
+  int hot_times;      // only deposits at observation indices below this go through the cache
   int n_prob;
   unsigned long long* work;  // per-problem particle counters (zeroed)
   unsigned int* active;      // per-problem count of work-groups on it (zeroed)
@@ -452,6 +462,9 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
   uint32_t* shist = (uint32_t*)(sguide + (p.guide_n > 0 ? p.ntab * (p.guide_n + 1) : 0));
   float* sw = (float*)shist;  // adjoint: the weight table takes the histogram's place
   constexpr bool ADJ = SINK == 1;
+  constexpr unsigned long long kEmpty = ~0ull;
+  unsigned long long* hkey = (unsigned long long*)((char*)smem + p.hot_off);  // HM_TMAJOR cache keys
+  uint32_t* hcnt = (uint32_t*)(hkey + p.hot_slots);                             // and counts
 
   if (p.edges_total > 0) {
     // gather the edge arrays into one LDS block
@@ -505,6 +518,12 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
         for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) sw[k] = P.w[k];
       } else {
         for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) shist[k] = 0u;
+      }
+    }
+    if (TMAJ && !ADJ) {
+      for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) {
+        hkey[k] = kEmpty;
+        hcnt[k] = 0u;
       }
     }
     __syncthreads();
@@ -743,7 +762,22 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
           if (TMAJ && !ADJ) {
             // time-major scratch: pair an even-time deposit with the next one
             const int64_t j = sp < 0 ? -1 : (sp + (isP ? 0 : p.time_stride)) * (int64_t)p.nt_pad + it;
-            if (tpend >= 0 && j == tpend + 1) {
+            bool cached = false;
+            if (j >= 0 && p.hot_slots > 0 && it < p.hot_times) {
+              const int s = (int)(((unsigned long long)j * 0x9E3779B97F4A7C15ull) >> p.hot_shift);
+              const unsigned long long key = hkey[s];
+              if (key == (unsigned long long)j) {
+                cached = true;
+              } else if (key == kEmpty) {
+                const unsigned long long old = atomicCAS(hkey + s, kEmpty, (unsigned long long)j);
+                cached = old == kEmpty || old == (unsigned long long)j;
+              }
+              if (cached) atomicAdd(hcnt + s, 1u);
+            }
+            if (cached) {
+              if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              tpend = -1;
+            } else if (tpend >= 0 && j == tpend + 1) {
               __hip_atomic_fetch_add((unsigned long long*)P.T + (tpend >> 1), 0x100000001ull, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
@@ -794,6 +828,13 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
 
     if (TMAJ && tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
+    if (TMAJ && !ADJ) {
+      for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) {
+        const unsigned long long key = hkey[k];
+        const uint32_t c = hcnt[k];
+        if (key != kEmpty && c) __hip_atomic_fetch_add(P.T + key, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     if (LDSH && !ADJ) {
       for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) {
         uint32_t c = shist[k];
