@@ -296,3 +296,48 @@ def test_energy_density_autograd_function(gpu):
     g = B.adjoint(R * rb.count_scales(O, Ns, 'cuda')).cpu()
     assert params.grad.shape == (2, 3)
     assert torch.allclose(params.grad, g, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("config,n", [(2, 10_000_000), (3, 2_000_000), (5, 10_000_000)])
+def test_full_size_histograms_placement_invariant(gpu, config, n):
+    """At the workloads' full histogram sizes (4.6 GB, 65 MB, 28.4 GB): the
+    time-major scratch with paired atomics and the LDS hot-counter cache gives
+    exactly the counts of direct global atomics."""
+    geometry, source, material, observation = getattr(cases, f"config{config}")(n=n)
+    d = geometry['dimension']
+    material = MaterialClass.prepareSigma(material, d)
+    O = initializeObservation(geometry, bool(material.acoustics), observation, n, allow_nd=True)
+    dm = solver.DeviceMaterial(material, d, 'cuda', with_zoeppritz=len(geometry.get('bnd', [])) > 0)
+    E1, c1, _k = solver.forward_histogram(geometry, source, material, O, n, seed=23, device='cuda', hist_mode=1,
+                                          dmat=dm)
+    E3, c3, _k = solver.forward_histogram(geometry, source, material, O, n, seed=23, device='cuda', hist_mode=3,
+                                          dmat=dm)
+    assert int(c1[1]) > 0
+    assert torch.equal(c1, c3)
+    assert torch.equal(E1, E3)
+    # every deposit is in the histogram
+    assert int(E1.sum(dtype=torch.int64)) == int(c1[1])
+
+
+def test_trace_timing_and_atomic_probe(gpu):
+    """Measurement aids: one timed launch per forward call; a plausible atomic rate."""
+    import ctypes as C
+    from radiativetransfermontecarlo_amd import _abi
+    lib = _abi.load()
+    geometry, source, base, O, mats = _acoustic(tmax=0.2)
+    _abi.check(lib.rtmc_set_trace_timing(1), "timing")
+    for k in range(3):
+        solver.forward_histogram(geometry, source, mats[0], O, 20_000, seed=k, device='cuda')
+    ms = (C.c_double * 8)()
+    n = C.c_int32()
+    _abi.check(lib.rtmc_trace_times(ms, 8, C.byref(n)), "times")
+    _abi.check(lib.rtmc_set_trace_timing(0), "timing")
+    assert n.value == 3 and all(0 < ms[i] < 1e4 for i in range(3))
+    buf = torch.zeros(1 << 20, dtype=torch.int32, device='cuda')
+    r = C.c_double()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for kind in range(4):
+        _abi.check(lib.rtmc_probe_atomics(C.c_void_p(buf.data_ptr()), 1 << 20, kind, 10, C.byref(r), stream),
+                   "probe")
+        assert 1e8 < r.value < 1e13
+    assert lib.rtmc_probe_atomics(C.c_void_p(buf.data_ptr()), 1 << 20, 7, 10, C.byref(r), stream) == _abi.ERR_ARG
