@@ -457,20 +457,30 @@ __device__ __forceinline__ void renorm(T d[3], T q[3]) {
 // Observation variables (observeTime.m:15-45): 0 x|r, 1 y|azimuth,
 // 2 z|sin(elevation), 3 direction.  `need` masks the ones to compute.
 // Computed in the flight precision T, returned in double for the binning.
-template <typename T>
+// Radii are hypot(hypot(x, y), z) as MATLAB's cart2pol / cart2sph compute
+// them; with FAST they are sqrt(x^2 + y^2 [+ z^2]), which differs by at most
+// an ulp or two and so moves a deposit only when the radius lies that close to
+// a bin edge (positions here are far from overflow or underflow).
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void obs_vars(int dim, int frame, const T X[3], T dz, unsigned need, double v[4]) {
   T r = T(0), th = T(0), ph = T(0);
   bool polar = frame != RTMC_FRAME_CARTESIAN;
+  auto r2 = [&](T a, T b) { return FAST ? sqrt(a * a + b * b) : hypot(a, b); };
   if (dim == 2) {
-    if (polar && (need & 1u)) r = hypot(X[0], X[1]);
+    if (polar && (need & 1u)) r = r2(X[0], X[1]);
     if (polar && (need & 2u)) th = atan2(X[1], X[0]);
   } else if (frame == RTMC_FRAME_CYLINDRICAL) {
-    if (need & 1u) r = hypot(X[0], X[1]);
+    if (need & 1u) r = r2(X[0], X[1]);
     if (need & 2u) th = atan2(X[1], X[0]);
   } else if (frame == RTMC_FRAME_SPHERICAL) {
     T hxy = T(0);
-    if (need & 5u) hxy = hypot(X[0], X[1]);
-    if (need & 1u) r = hypot(hxy, X[2]);
+    if (FAST) {
+      if (need & 4u) hxy = sqrt(X[0] * X[0] + X[1] * X[1]);
+      if (need & 1u) r = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    } else {
+      if (need & 5u) hxy = hypot(X[0], X[1]);
+      if (need & 1u) r = hypot(hxy, X[2]);
+    }
     if (need & 2u) th = atan2(X[1], X[0]);
     if (need & 4u) ph = sin(atan2(X[2], hxy));
   }

@@ -140,13 +140,17 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   for (int v = 0; v < 4; ++v) {
     p.edges[v] = B.edges[v];
     p.ne[v] = B.n_edges[v];
-    p.stride[v] = B.stride[v];
+    RTMC_REQUIRE(B.stride[v] >= 0 && B.stride[v] < ((int64_t)1 << 31), "bin stride %d out of range", v + 1);
+    p.stride[v] = (uint32_t)B.stride[v];
     p.eoff[v] = total;
     if (dim == 2 && v == 2) continue;
     p.present |= 1u << v;
     total += B.n_edges[v];
   }
+  RTMC_REQUIRE(B.time_stride >= 1 && B.time_stride < ((int64_t)1 << 31),
+               "time_stride must be in [1, 2^31): one time slice of at most 2^31 - 1 bins");
   p.time_stride = B.time_stride;
+  p.time_stride32 = (uint32_t)B.time_stride;
   p.mode_stride = B.mode_stride;
   p.n_times = n_times;
   p.dt_obs = dt;

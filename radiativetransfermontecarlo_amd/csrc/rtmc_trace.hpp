@@ -112,9 +112,10 @@ struct LaunchParams {
   const double* edges[4];
   int ne[4];
   int eoff[4];        // offset of each edge array in the LDS copy
-  int64_t stride[4];
+  uint32_t stride[4]; // bin strides (a bin index within one time slice is < time_stride < 2^31)
   unsigned present;   // variables that exist (bit v; z is absent in 2-D)
   int64_t time_stride;
+  uint32_t time_stride32;  // the same, for 32 x 32 -> 64-bit products
   int64_t mode_stride;
   int n_times;
   double dt_obs;
@@ -382,14 +383,14 @@ __device__ __forceinline__ int dir_bin(const LaunchParams& p, const BinSetup& bs
 // + histcounts), without the time and mode terms, or -1 when outside; the
 // direction's bin `dbin` comes from dir_bin.
 template <typename R>
-__device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSetup& bs,
-                                             const double* __restrict__ sedges, int dim, const R X[3], int dbin) {
+__device__ __forceinline__ int bin_index(const LaunchParams& p, const BinSetup& bs,
+                                         const double* __restrict__ sedges, int dim, const R X[3], int dbin) {
   double v[4];
-  obs_vars(dim, p.G.frame, X, R(0), bs.check & 7u, v);
-  int64_t idx = 0;
+  obs_vars<R, true>(dim, p.G.frame, X, R(0), bs.check & 7u, v);
+  uint32_t idx = 0;
   if (bs.check & 8u) {
     if (dbin < 0) return -1;
-    idx = (int64_t)dbin * p.stride[3];
+    idx = (uint32_t)dbin * p.stride[3];
   }
   if (p.edges_total > 0) {
     // edges in LDS (`sedges` points into the work-group's LDS block)
@@ -398,7 +399,7 @@ __device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSet
       if (!(bs.check & (1u << k))) continue;
       int b = bin_guess(sedges + p.eoff[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
       if (b < 0) return -1;
-      idx += (int64_t)b * p.stride[k];
+      idx += (uint32_t)b * p.stride[k];
     }
   } else {
 #pragma unroll
@@ -406,10 +407,10 @@ __device__ __forceinline__ int64_t bin_index(const LaunchParams& p, const BinSet
       if (!(bs.check & (1u << k))) continue;
       int b = bin_guess(p.edges[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
       if (b < 0) return -1;
-      idx += (int64_t)b * p.stride[k];
+      idx += (uint32_t)b * p.stride[k];
     }
   }
-  return idx;
+  return (int)idx;
 }
 
 // d log sigma / d lc at theta (linear interpolation on the uniform grid)
@@ -757,11 +758,16 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             const R f = (R)((double)h * ((double)it * p.dt_obs - tb));
             for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
           }
-          const int64_t sp = bin_index(p, bs, sedges, DIM, X, dbin);
-          const int64_t idx = sp < 0 ? -1 : sp + (int64_t)it * p.time_stride + (isP ? 0 : p.mode_stride);
+          const int sp = bin_index(p, bs, sedges, DIM, X, dbin);
+          const int64_t idx = sp < 0 ? -1
+                                     : (int64_t)sp + (uint64_t)(uint32_t)it * p.time_stride32 +
+                                           (isP ? 0 : p.mode_stride);
           if (TMAJ && !ADJ) {
             // time-major scratch: pair an even-time deposit with the next one
-            const int64_t j = sp < 0 ? -1 : (sp + (isP ? 0 : p.time_stride)) * (int64_t)p.nt_pad + it;
+            const int64_t j = sp < 0 ? -1
+                                     : (int64_t)((uint64_t)((uint32_t)sp + (isP ? 0u : p.time_stride32)) *
+                                                     (uint32_t)p.nt_pad +
+                                                 (uint32_t)it);
             bool cached = false;
             if (j >= 0 && p.hot_slots > 0 && it < p.hot_times) {
               const int s = (int)(((unsigned long long)j * 0x9E3779B97F4A7C15ull) >> p.hot_shift);
