@@ -529,4 +529,19 @@ __device__ __forceinline__ int bin_guess(const double* __restrict__ e, int ne, d
   return k;
 }
 
+// bin_guess with the last edge `ehi` known: the guessed bin's two edges are read
+// together and usually confirm it; otherwise the search walks as bin_guess does.
+__device__ __forceinline__ int bin_guess2(const double* __restrict__ e, int ne, double e0, double ehi, double inv_w,
+                                          double v) {
+  if (!(v >= e0 && v <= ehi)) return -1;
+  const int nb = ne - 1;
+  const double g = (v - e0) * inv_w;
+  int k = (g >= 0.0 && g < (double)nb) ? (int)g : (g >= (double)nb ? nb - 1 : 0);
+  const double lo = e[k], hi = e[k + 1];
+  if (v >= lo && (v < hi || k == nb - 1)) return k;
+  while (k > 0 && v < e[k]) --k;
+  while (k < nb - 1 && v >= e[k + 1]) ++k;
+  return k;
+}
+
 }  // namespace rtmc

@@ -240,6 +240,7 @@ __device__ __forceinline__ int64_t predict_boundary(const rtmc_boundary& B, cons
 struct BinSetup {
   unsigned check;  // variables to bin / range-check (bit v)
   double e0[4];
+  double ehi[4];   // last edge
   double invw[4];
 };
 
@@ -260,6 +261,7 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
   bs->check = 0u;
   for (int v = 0; v < 4; ++v) {
     bs->e0[v] = 0.0;
+    bs->ehi[v] = 0.0;
     bs->invw[v] = 0.0;
     if (!(p.present & (1u << v))) continue;
     const double* e = sedges ? sedges + p.eoff[v] : p.edges[v];
@@ -267,6 +269,7 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
     if (p.ne[v] == 2 && trivially_inside(v, dim, p.G.frame, lo, hi)) continue;
     bs->check |= 1u << v;
     bs->e0[v] = lo;
+    bs->ehi[v] = hi;
     double w = hi - lo;
     bs->invw[v] = (isfinite(w) && w > 0.0) ? (double)(p.ne[v] - 1) / w : 0.0;
   }
@@ -376,7 +379,7 @@ __device__ __forceinline__ int dir_bin(const LaunchParams& p, const BinSetup& bs
   if (cp < R(-1)) cp = R(-1);
   const R psi = (dim == 3) ? cp : acos(cp);
   const double* e = p.edges_total > 0 ? sedges + p.eoff[3] : p.edges[3];
-  return bin_guess(e, p.ne[3], bs.e0[3], bs.invw[3], (double)psi);
+  return bin_guess2(e, p.ne[3], bs.e0[3], bs.ehi[3], bs.invw[3], (double)psi);
 }
 
 // Index of the observation's bin over the binned variables (observeTime.m:15-45
@@ -397,7 +400,7 @@ __device__ __forceinline__ int bin_index(const LaunchParams& p, const BinSetup& 
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (!(bs.check & (1u << k))) continue;
-      int b = bin_guess(sedges + p.eoff[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
+      int b = bin_guess2(sedges + p.eoff[k], p.ne[k], bs.e0[k], bs.ehi[k], bs.invw[k], v[k]);
       if (b < 0) return -1;
       idx += (uint32_t)b * p.stride[k];
     }
@@ -405,7 +408,7 @@ __device__ __forceinline__ int bin_index(const LaunchParams& p, const BinSetup& 
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (!(bs.check & (1u << k))) continue;
-      int b = bin_guess(p.edges[k], p.ne[k], bs.e0[k], bs.invw[k], v[k]);
+      int b = bin_guess2(p.edges[k], p.ne[k], bs.e0[k], bs.ehi[k], bs.invw[k], v[k]);
       if (b < 0) return -1;
       idx += (uint32_t)b * p.stride[k];
     }
