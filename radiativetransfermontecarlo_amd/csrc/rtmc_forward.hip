@@ -537,9 +537,13 @@ extern "C" int rtmc_autotune_batched(const rtmc_batch* batch, double sample_frac
       for (int ev : {8, 16, 24, 32})
         for (int ppt : {0, 4}) cands.push_back({b, ppt, h, ev});
   }
-  hipEvent_t e0, e1;
-  RTMC_HIP(hipEventCreate(&e0));
-  RTMC_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_error("autotune: hipEventCreate failed");
+    if (e0) (void)hipEventDestroy(e0);
+    (void)hipFreeAsync(scratch, st);
+    return RTMC_ERR_HIP;
+  }
   auto time_one = [&](const Cand* c, double* ms) -> int {
     rtmc_batch tb = sb;
     if (c) {
