@@ -375,6 +375,64 @@ typedef struct {
 int rtmc_autotune_batched(const rtmc_batch* batch, double sample_fraction,
                           rtmc_tuning* best, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* Native material preparation: MaterialClass.prepareSigma
+ * (MaterialClass.m:2379-2463) with calcSigmaIsotropic (:172-359) and the
+ * PSDF laws (:381-710) for isotropic random media. */
+
+/* MaterialClass.SpectralLaw */
+#define RTMC_PSDF_EXP 0
+#define RTMC_PSDF_POWER_LAW 1
+#define RTMC_PSDF_GAUSSIAN 2
+#define RTMC_PSDF_TRIANGULAR 3    /* 3-D only */
+#define RTMC_PSDF_LOW_PASS 4      /* 3-D only */
+#define RTMC_PSDF_VON_KARMAN 5
+
+/* The MaterialClass properties prepareSigma reads. */
+typedef struct {
+    int32_t dimension;          /* 2 or 3 */
+    int32_t acoustics;          /* 1 acoustic, 0 elastic */
+    double v[2];                /* acoustic: v in [0]; elastic: vp, vs */
+    double frequency;           /* Frequency */
+    double cv[3];               /* coefficients_of_variation: acoustic kappa, rho; elastic lambda, mu, rho */
+    double corr[3];             /* correlation_coefficients: acoustic [kappa-rho];
+                                   elastic lambda-mu, lambda-rho, mu-rho */
+    int32_t spectral_law;       /* RTMC_PSDF_* */
+    int32_t pad_;
+    double lc;                  /* CorrelationLength */
+    double nu;                  /* SpectralParam.nu (von Karman) */
+    double Q[2];                /* QP, QS; +Inf = no attenuation (acoustic: Q[0]) */
+    double knot_threshold;      /* 1e-8: the reference's tables; 0: every strictly increasing knot,
+                                   held beyond the last one (what the adjoint's score assumes) */
+    int64_t n_theta;            /* angle grid of the pdf; 0 = 1e6 as prepareSigmaOne */
+} rtmc_material_params;
+
+/* Host-side result of rtmc_prepare_material_host (arrays malloc'ed by the
+ * library; rtmc_free_material_host releases them). */
+typedef struct {
+    double Sigma[4];            /* acoustic: Sigma in [0]; elastic: PP, PS, SP, SS */
+    double Sigma_prime[4];      /* Sigmapr, same order */
+    double mft[2];              /* meanFreeTime of P and S */
+    double P2P, S2S;
+    int64_t n_knots[4];         /* per table: [0] acoustic, or PP, PS, SP, SS */
+    double* x[4];               /* cdf knots (GridVectors{1}) */
+    double* y[4];               /* angles (Values) */
+    int32_t extrap;             /* RTMC_EXTRAP_* */
+    int32_t pad_;
+} rtmc_material_host;
+
+/* Replaces MaterialClass.prepareSigma for isotropic media: Sigma, the mean
+ * free times, P2P / S2S and the inverse-CDF tables, on the host. */
+int rtmc_prepare_material_host(const rtmc_material_params* params, rtmc_material_host* out);
+void rtmc_free_material_host(rtmc_material_host* out);
+
+/* The same, uploaded: fills `out` (device tables) for rtmc_forward and the
+ * batch entry points; *handle owns the device memory until
+ * rtmc_free_material.  Zoeppritz tables (elastic reflective boundaries) are
+ * not computed here (out->n_zoe = 0). */
+int rtmc_prepare_material(const rtmc_material_params* params, rtmc_material* out, void** handle);
+int rtmc_free_material(void* handle);
+
 /* Measurement aid, no counterpart in the reference: throughput of no-return
  * global atomics (requests/s) on the current device, the ceiling of the
  * fused kernel's deposits when the histogram is in global memory.  buf:

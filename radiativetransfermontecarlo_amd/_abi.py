@@ -226,6 +226,42 @@ class Tuning(C.Structure):
     ]
 
 
+PSDF = {"exp": 0, "power_law": 1, "gaussian": 2, "triangular": 3, "low_pass": 4, "VonKarman": 5}
+
+
+class MaterialParams(C.Structure):
+    _fields_ = [
+        ("dimension", C.c_int32),
+        ("acoustics", C.c_int32),
+        ("v", C.c_double * 2),
+        ("frequency", C.c_double),
+        ("cv", C.c_double * 3),
+        ("corr", C.c_double * 3),
+        ("spectral_law", C.c_int32),
+        ("pad_", C.c_int32),
+        ("lc", C.c_double),
+        ("nu", C.c_double),
+        ("Q", C.c_double * 2),
+        ("knot_threshold", C.c_double),
+        ("n_theta", C.c_int64),
+    ]
+
+
+class MaterialHost(C.Structure):
+    _fields_ = [
+        ("Sigma", C.c_double * 4),
+        ("Sigma_prime", C.c_double * 4),
+        ("mft", C.c_double * 2),
+        ("P2P", C.c_double),
+        ("S2S", C.c_double),
+        ("n_knots", C.c_int64 * 4),
+        ("x", C.POINTER(C.c_double) * 4),
+        ("y", C.POINTER(C.c_double) * 4),
+        ("extrap", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
 class SubstepInfo(C.Structure):
     _fields_ = [
         ("T_eff", C.c_double),
@@ -251,6 +287,10 @@ EXPORTS = (
     "rtmc_forward_batched",
     "rtmc_adjoint_batched",
     "rtmc_autotune_batched",
+    "rtmc_prepare_material_host",
+    "rtmc_free_material_host",
+    "rtmc_prepare_material",
+    "rtmc_free_material",
     "rtmc_probe_atomics",
     "rtmc_set_trace_timing",
     "rtmc_trace_times",
@@ -287,6 +327,14 @@ def _declare(lib):
     lib.rtmc_adjoint_batched.argtypes = [P(Batch), P(AdjointItem), C.c_void_p]
     lib.rtmc_autotune_batched.restype = C.c_int
     lib.rtmc_autotune_batched.argtypes = [P(Batch), C.c_double, P(Tuning), C.c_void_p]
+    lib.rtmc_prepare_material_host.restype = C.c_int
+    lib.rtmc_prepare_material_host.argtypes = [P(MaterialParams), P(MaterialHost)]
+    lib.rtmc_free_material_host.restype = None
+    lib.rtmc_free_material_host.argtypes = [P(MaterialHost)]
+    lib.rtmc_prepare_material.restype = C.c_int
+    lib.rtmc_prepare_material.argtypes = [P(MaterialParams), P(Material), P(C.c_void_p)]
+    lib.rtmc_free_material.restype = C.c_int
+    lib.rtmc_free_material.argtypes = [C.c_void_p]
     lib.rtmc_set_trace_timing.restype = C.c_int
     lib.rtmc_set_trace_timing.argtypes = [C.c_int32]
     lib.rtmc_trace_times.restype = C.c_int

@@ -1,0 +1,414 @@
+// rtmc_material.hip -- rtmc_prepare_material: the scattering operator of
+// MaterialClass (cross-sections, total cross-sections, inverse-CDF tables of
+// the scattering angle) computed natively on the host and uploaded for the
+// kernels.  Restates MaterialClass.m:172-359 (calcSigmaIsotropic),
+// :381-710 (PSDF laws), :2379-2463 (prepareSigma, prepareSigmaOne).
+#include "rtmc_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace rtmc {
+namespace {
+
+constexpr double kPiH = 3.141592653589793;
+
+// H(0) = 1/2 (MaterialClass.heaviside, MaterialClass.m:1760-1764)
+double heaviside(double h) { return h > 0.0 ? 1.0 : (h == 0.0 ? 0.5 : 0.0); }
+
+// Power spectral density Phi(z) of the correlation law, z = wavenumber * lc
+// (MaterialClass.m:479-710).
+struct Psdf {
+  int law, d;
+  double a, nu;
+  double operator()(double z) const {
+    switch (law) {
+      case RTMC_PSDF_EXP:
+        if (d == 2) return 1.0 / (4.0 * kPiH * std::pow(1.0 + (z / a) * (z / a), 1.5));
+        return 1.0 / (6.0 * (kPiH * kPiH) * std::pow(1.0 + (z / a) * (z / a), 2.0));
+      case RTMC_PSDF_POWER_LAW:
+        if (d == 2) return z * std::cyl_bessel_k(1.0, z) / (4.0 * kPiH);
+        return 1.0 / (6.0 * (kPiH * kPiH)) * std::exp(-z / a);
+      case RTMC_PSDF_GAUSSIAN:
+        if (d == 2) return 1.0 / (4.0 * kPiH) * std::exp(-(z * z) / 4.0);
+        return 1.0 / (6.0 * (kPiH * kPiH)) * std::exp(-(z * z) / (4.0 * a));
+      case RTMC_PSDF_TRIANGULAR:
+        return (3.0 / 8.0 / std::pow(kPiH, 4.0)) * (1.0 - z / 2.0 / kPiH) * heaviside(2.0 * kPiH - z);
+      case RTMC_PSDF_LOW_PASS:
+        return (2.0 / 9.0 / std::pow(kPiH, 4.0)) * heaviside(a - z);
+      default:  // von Karman
+        if (d == 2) return 1.0 / (4.0 * kPiH) / std::pow(1.0 + (z / a) * (z / a), nu + 1.0);
+        return 1.0 / (6.0 * (kPiH * kPiH)) / std::pow(1.0 + (z / a) * (z / a), nu + 1.5);
+    }
+  }
+};
+
+int make_psdf(const rtmc_material_params& p, Psdf* out) {
+  Psdf P;
+  P.law = p.spectral_law;
+  P.d = p.dimension;
+  P.nu = p.nu;
+  P.a = 0.0;
+  switch (p.spectral_law) {
+    case RTMC_PSDF_EXP:
+      P.a = p.dimension == 2 ? std::sqrt(2.0) : std::pow(6.0, 1.0 / 3.0);
+      break;
+    case RTMC_PSDF_POWER_LAW:
+      P.a = std::pow(3.0 * kPiH / 4.0, 1.0 / 3.0);
+      break;
+    case RTMC_PSDF_GAUSSIAN:
+      P.a = std::pow(3.0 * std::sqrt(kPiH) / 4.0, 2.0 / 3.0);
+      break;
+    case RTMC_PSDF_TRIANGULAR:
+      RTMC_REQUIRE(p.dimension == 3, "dimensions other than 3 are not coded yet!");
+      break;
+    case RTMC_PSDF_LOW_PASS:
+      RTMC_REQUIRE(p.dimension == 3, "LowPass spectral model is currently implemented only for d = 3.");
+      P.a = 3.0 * kPiH / 2.0;
+      break;
+    case RTMC_PSDF_VON_KARMAN:
+      RTMC_REQUIRE(p.nu > 0.0, "Please add the Hurst number for VonKarman PSDF");
+      P.a = p.dimension == 2 ? 2.0 * std::sqrt(p.nu)
+                             : std::pow(6.0 * std::sqrt(kPiH) * std::tgamma(p.nu + 1.5) / std::tgamma(p.nu), 1.0 / 3.0);
+      break;
+    default:
+      set_error("unknown spectral law %d", p.spectral_law);
+      return RTMC_ERR_ARG;
+  }
+  *out = P;
+  return RTMC_OK;
+}
+
+// Differential scattering cross-sections (calcSigmaIsotropic, MaterialClass.m:172-359):
+// channel 0 (acoustic) or 0..3 = PP, PS, SP, SS (elastic).
+struct Sigma {
+  Psdf phi;
+  int d, acoustics, channel;
+  double omega, zeta, zP, zS, K;
+  double dkk, drr, rkr;            // acoustic
+  double dll, dmm, rlm, rlr, rmr;  // elastic (drr shared)
+  double operator()(double th) const {
+    const double c = std::cos(th);
+    if (acoustics) {
+      const double p = d == 2 ? 2.0 : 3.0;
+      return (kPiH / 2.0) * omega * std::pow(zeta, p) * (c * c * (drr * drr) + 2.0 * c * rkr * dkk * drr + dkk * dkk) *
+             phi(zeta * std::sqrt(2.0 * (1.0 - c)));
+    }
+    const double K2 = K * K, K4 = std::pow(K, 4.0);
+    if (d == 2) {
+      switch (channel) {
+        case 0:
+          return (kPiH / 2.0) * omega * (zP * zP) *
+                 ((1.0 - 2.0 / K2) * (1.0 - 2.0 / K2) * (dll * dll) + 4.0 * (1.0 / K2 - 2.0 / K4) * rlm * dll * dmm * (c * c) +
+                  (4.0 / K4) * (dmm * dmm) * std::pow(c, 4.0) + (drr * drr) * (c * c) -
+                  2.0 * (1.0 - 2.0 / K2) * rlr * dll * drr * c - (4.0 / K2) * rmr * dmm * drr * std::pow(c, 3.0)) *
+                 phi(zP * std::sqrt(2.0 * (1.0 - c)));
+        case 1:
+          return (kPiH / 2.0) * omega * (zP * zP) *
+                 (K2 * (drr * drr) + 4.0 * (dmm * dmm) * (c * c) - 4.0 * K * rmr * dmm * drr * c) * (1.0 - c * c) *
+                 phi(zP * std::sqrt(1.0 + K2 - 2.0 * K * c));
+        case 2:
+          return (kPiH / 2.0) * omega * (zP * zP) *
+                 ((drr * drr) + (4.0 / K2) * (dmm * dmm) * (c * c) - (4.0 / K) * rmr * dmm * drr * c) * (1.0 - c * c) *
+                 phi(zS * std::sqrt(1.0 + 1.0 / K2 - 2.0 * c / K));
+        default: {
+          const double G = 2.0 * (c * c) - 1.0;
+          return (kPiH / 2.0) * omega * (zS * zS) *
+                 ((drr * drr) * (c * c) + (dmm * dmm) * (G * G) - 2.0 * rmr * dmm * drr * c * G) *
+                 phi(zS * std::sqrt(2.0 * (1.0 - c)));
+        }
+      }
+    }
+    switch (channel) {
+      case 0:
+        return (kPiH / 2.0) * omega * std::pow(zP, 3.0) *
+               ((1.0 - 2.0 / K2) * (1.0 - 2.0 / K2) * (dll * dll) + 4.0 * (1.0 / K2 - 2.0 / K4) * rlm * dll * dmm * (c * c) +
+                (4.0 / K4) * (dmm * dmm) * std::pow(c, 4.0) + (drr * drr) * (c * c) -
+                2.0 * (1.0 - 2.0 / K2) * rlr * dll * drr * c - (4.0 / K2) * rmr * dmm * drr * std::pow(c, 3.0)) *
+               phi(zP * std::sqrt(2.0 * (1.0 - c)));
+      case 1:
+        return (kPiH / 2.0) * K * omega * std::pow(zP, 3.0) *
+               (K2 * (drr * drr) + 4.0 * (dmm * dmm) * (c * c) - 4.0 * K * rmr * dmm * drr * c) * (1.0 - c * c) *
+               phi(zP * std::sqrt(1.0 + K2 - 2.0 * K * c));
+      case 2:
+        return (kPiH / 4.0 / std::pow(K, 3.0)) * omega * std::pow(zS, 3.0) *
+               ((drr * drr) + (4.0 / K2) * (dmm * dmm) * (c * c) - (4.0 / K) * rmr * dmm * drr * c) * (1.0 - c * c) *
+               phi(zS * std::sqrt(1.0 + 1.0 / K2 - 2.0 / K * c));
+      default: {
+        const double ph = phi(zS * std::sqrt(2.0 * (1.0 - c)));
+        const double tt = (kPiH / 4.0) * omega * std::pow(zS, 3.0) * (drr * drr) * (1.0 + c * c) * ph;
+        const double gg = (kPiH / 4.0) * omega * std::pow(zS, 3.0) * (dmm * dmm) *
+                          (4.0 * std::pow(c, 4.0) - 3.0 * (c * c) + 1.0) * ph;
+        const double gt = -(kPiH / 4.0) * omega * std::pow(zS, 3.0) * rmr * dmm * drr * (4.0 * std::pow(c, 3.0)) * ph;
+        return tt + gg + gt;
+      }
+    }
+  }
+};
+
+// Gauss-Legendre, 20 nodes on [-1, 1] (positive half; the rule is symmetric)
+const double kGLx[10] = {0.07652652113349734, 0.2277858511416451, 0.37370608871541955, 0.5108670019508271,
+                         0.636053680726515,   0.7463319064601508, 0.8391169718222188,  0.9122344282513258,
+                         0.9639719272779138,  0.9931285991850949};
+const double kGLw[10] = {0.15275338713072578, 0.14917298647260366, 0.14209610931838187, 0.13168863844917653,
+                         0.11819453196151825, 0.10193011981724026, 0.08327674157670467, 0.06267204833410944,
+                         0.04060142980038622, 0.017614007139153273};
+
+template <class F>
+double gl20(const F& f, double a, double b) {
+  const double m = 0.5 * (a + b), h = 0.5 * (b - a);
+  double s = 0.0;
+  for (int i = 0; i < 10; ++i) s += kGLw[i] * (f(m - h * kGLx[i]) + f(m + h * kGLx[i]));
+  return s * h;
+}
+
+// Adaptive bisection of 20-point Gauss-Legendre: an interval is accepted when
+// its rule and the sum over its halves agree to 1e-14 of the total (the
+// integral of MaterialClass.m, at a tighter tolerance than MATLAB's 1e-6).
+template <class F>
+double integral(const F& f, double a, double b) {
+  const int n0 = 16;
+  double est = 0.0;
+  struct Iv {
+    double a, b, v;
+    int depth;
+  };
+  std::vector<Iv> stack;
+  for (int i = 0; i < n0; ++i) {
+    double x0 = a + (b - a) * i / n0, x1 = a + (b - a) * (i + 1) / n0;
+    double v = gl20(f, x0, x1);
+    est += v;
+    stack.push_back({x0, x1, v, 0});
+  }
+  const double tol = 1e-14 * std::fabs(est) + 1e-300;
+  double total = 0.0;
+  while (!stack.empty()) {
+    Iv iv = stack.back();
+    stack.pop_back();
+    const double m = 0.5 * (iv.a + iv.b);
+    const double l = gl20(f, iv.a, m), r = gl20(f, m, iv.b);
+    if (std::fabs(l + r - iv.v) <= tol || iv.depth >= 50) {
+      total += l + r;
+    } else {
+      stack.push_back({iv.a, m, l, iv.depth + 1});
+      stack.push_back({m, iv.b, r, iv.depth + 1});
+    }
+  }
+  return total;
+}
+
+// prepareSigmaOne (MaterialClass.m:2443-2463): Sigma, Sigma' and the inverse
+// CDF of the scattering angle on n points of [0, pi].
+void prepare_one(const Sigma& sg, int d, int64_t n, double thr, double* S, double* Sp, std::vector<double>* kx,
+                 std::vector<double>* ky) {
+  if (d == 2) {
+    *S = 2.0 * integral(sg, 0.0, kPiH);
+    *Sp = 2.0 * integral([&](double t) { return sg(t) * std::cos(t); }, 0.0, kPiH);
+  } else {
+    *S = 2.0 * kPiH * integral([&](double t) { return sg(t) * std::sin(t); }, 0.0, kPiH);
+    *Sp = 2.0 * kPiH * integral([&](double t) { return sg(t) * std::sin(t) * std::cos(t); }, 0.0, kPiH);
+  }
+  const double step = kPiH / (double)(n - 1);
+  std::vector<double> cdf((size_t)n);
+  // pdf on the grid in parallel chunks, then the running sum in order (nancumsum)
+  {
+    const double Sv = *S;
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const int64_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+      th.emplace_back([&, t]() {
+        const int64_t i0 = (int64_t)t * per, i1 = std::min(n, i0 + per);
+        for (int64_t i = i0; i < i1; ++i) {
+          const double x = (i == n - 1) ? kPiH : (double)i * step;
+          cdf[(size_t)i] = (d == 2) ? (2.0 / Sv) * sg(x) : (2.0 * kPiH / Sv) * std::sin(x) * sg(x);
+        }
+      });
+    }
+    for (auto& t : th) t.join();
+  }
+  double run = 0.0;
+  const double dx = kPiH / (double)n;
+  for (int64_t i = 0; i < n; ++i) {
+    const double pdf = cdf[(size_t)i];
+    if (!std::isnan(pdf)) run += pdf;
+    cdf[(size_t)i] = run * dx;
+  }
+  // knots i where the cdf step before or after exceeds the threshold
+  kx->clear();
+  ky->clear();
+  for (int64_t i = 0; i < n; ++i) {
+    const bool after = i + 1 < n && (cdf[(size_t)i + 1] - cdf[(size_t)i]) > thr;
+    const bool before = i > 0 && (cdf[(size_t)i] - cdf[(size_t)i - 1]) > thr;
+    if (!(after || before)) continue;
+    const double th = (i == n - 1) ? kPiH : (double)i * step;
+    if (thr <= 0.0 && !kx->empty() && !(cdf[(size_t)i] > kx->back())) continue;  // repeated knot
+    kx->push_back(cdf[(size_t)i]);
+    ky->push_back(th);
+  }
+}
+
+struct DeviceTables {
+  std::vector<void*> ptrs;
+};
+
+}  // namespace
+}  // namespace rtmc
+
+using namespace rtmc;
+
+extern "C" int rtmc_prepare_material_host(const rtmc_material_params* p, rtmc_material_host* out) {
+  clear_error();
+  RTMC_REQUIRE(p != nullptr && out != nullptr, "params and out must be given");
+  memset(out, 0, sizeof(*out));
+  RTMC_REQUIRE(p->dimension == 2 || p->dimension == 3, "dimension must be 2 or 3 (sigma for 1D not implemented)");
+  RTMC_REQUIRE(p->frequency > 0.0 && p->lc > 0.0, "frequency and correlation length must be > 0");
+  RTMC_REQUIRE(p->acoustics ? p->v[0] > 0.0 : (p->v[0] > 0.0 && p->v[1] > 0.0), "velocities must be > 0");
+  if (p->acoustics) {
+    RTMC_REQUIRE(std::fabs(p->corr[0]) <= 1.0, "Absolute value of corr(kappa,rho) should be less than 1!");
+  } else {
+    for (int k = 0; k < 3; ++k)
+      RTMC_REQUIRE(std::fabs(p->corr[k]) <= 1.0, "Absolute values of correlation coefficients should be less than 1!");
+  }
+  const int64_t n = p->n_theta > 0 ? p->n_theta : 1000000;
+  RTMC_REQUIRE(n >= 16, "n_theta must be >= 16");
+  Psdf phi;
+  int e = make_psdf(*p, &phi);
+  if (e) return e;
+  Sigma base;
+  base.phi = phi;
+  base.d = p->dimension;
+  base.acoustics = p->acoustics;
+  base.channel = 0;
+  base.omega = 2.0 * kPiH * p->frequency;
+  if (p->acoustics) {
+    base.zeta = base.omega / p->v[0] * p->lc;  // MaterialClass.m:204
+    base.dkk = p->cv[0];
+    base.drr = p->cv[1];
+    base.rkr = p->corr[0];
+  } else {
+    base.K = p->v[0] / p->v[1];  // :242-253
+    base.zP = base.omega / p->v[0] * p->lc;
+    base.zS = base.K * base.zP;
+    base.dll = p->cv[0];
+    base.dmm = p->cv[1];
+    base.drr = p->cv[2];
+    base.rlm = p->corr[0];
+    base.rlr = p->corr[1];
+    base.rmr = p->corr[2];
+  }
+  const int nch = p->acoustics ? 1 : 4;
+  std::vector<double> kx[4], ky[4];
+  double S[4] = {0, 0, 0, 0}, Sp[4] = {0, 0, 0, 0};
+  {
+    std::vector<std::thread> th;
+    for (int c = 0; c < nch; ++c) {
+      th.emplace_back([&, c]() {
+        Sigma sg = base;
+        sg.channel = c;
+        prepare_one(sg, p->dimension, n, p->knot_threshold, &S[c], &Sp[c], &kx[c], &ky[c]);
+      });
+    }
+    for (auto& t : th) t.join();
+  }
+  for (int c = 0; c < nch; ++c) {
+    if (kx[c].size() < 2) {
+      set_error("inverse CDF of channel %d has %zu knots", c, kx[c].size());
+      return RTMC_ERR_TABLE;
+    }
+    out->Sigma[c] = S[c];
+    out->Sigma_prime[c] = Sp[c];
+    out->n_knots[c] = (int64_t)kx[c].size();
+    out->x[c] = (double*)malloc(sizeof(double) * kx[c].size());
+    out->y[c] = (double*)malloc(sizeof(double) * kx[c].size());
+    if (!out->x[c] || !out->y[c]) {
+      rtmc_free_material_host(out);
+      set_error("out of host memory");
+      return RTMC_ERR_NOMEM;
+    }
+    memcpy(out->x[c], kx[c].data(), sizeof(double) * kx[c].size());
+    memcpy(out->y[c], ky[c].data(), sizeof(double) * kx[c].size());
+  }
+  out->extrap = p->knot_threshold <= 0.0 ? RTMC_EXTRAP_NEAREST : RTMC_EXTRAP_LINEAR;
+  if (p->acoustics) {
+    out->mft[0] = out->mft[1] = 1.0 / S[0];  // MaterialClass.m:2385
+    out->P2P = out->S2S = 1.0;
+  } else {
+    out->mft[0] = 1.0 / (S[0] + S[1]);  // :2412
+    out->mft[1] = 1.0 / (S[2] + S[3]);
+    out->P2P = S[0] / (S[0] + S[1]);    // :2436-2437
+    out->S2S = S[3] / (S[3] + S[2]);
+  }
+  return RTMC_OK;
+}
+
+extern "C" void rtmc_free_material_host(rtmc_material_host* out) {
+  if (!out) return;
+  for (int c = 0; c < 4; ++c) {
+    free(out->x[c]);
+    free(out->y[c]);
+    out->x[c] = out->y[c] = nullptr;
+    out->n_knots[c] = 0;
+  }
+}
+
+extern "C" int rtmc_prepare_material(const rtmc_material_params* p, rtmc_material* out, void** handle) {
+  clear_error();
+  RTMC_REQUIRE(out != nullptr && handle != nullptr, "out and handle must be given");
+  *handle = nullptr;
+  rtmc_material_host h;
+  int e = rtmc_prepare_material_host(p, &h);
+  if (e) return e;
+  DeviceTables* dt = new DeviceTables();
+  memset(out, 0, sizeof(*out));
+  out->acoustics = p->acoustics;
+  out->vp = p->v[0];
+  out->vs = p->acoustics ? 0.0 : p->v[1];
+  out->mft[0] = h.mft[0];
+  out->mft[1] = h.mft[1];
+  out->P2P = h.P2P;
+  out->S2S = h.S2S;
+  out->QP = p->Q[0];
+  out->QS = p->acoustics ? p->Q[0] : p->Q[1];
+  out->frequency = p->frequency;
+  const int nch = p->acoustics ? 1 : 4;
+  int rc = RTMC_OK;
+  for (int c = 0; c < nch && rc == RTMC_OK; ++c) {
+    const size_t bytes = sizeof(double) * (size_t)h.n_knots[c];
+    void* dx = nullptr;
+    void* dy = nullptr;
+    if (hipMalloc(&dx, bytes) != hipSuccess || hipMalloc(&dy, bytes) != hipSuccess ||
+        hipMemcpy(dx, h.x[c], bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy, h.y[c], bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("uploading the inverse-CDF table failed");
+      rc = RTMC_ERR_NOMEM;
+    }
+    if (dx) dt->ptrs.push_back(dx);
+    if (dy) dt->ptrs.push_back(dy);
+    out->invcdf[c].n = h.n_knots[c];
+    out->invcdf[c].x = (const double*)dx;
+    out->invcdf[c].y = (const double*)dy;
+    out->invcdf[c].extrap = h.extrap;
+  }
+  rtmc_free_material_host(&h);
+  if (rc != RTMC_OK) {
+    for (void* q : dt->ptrs) (void)hipFree(q);
+    delete dt;
+    return rc;
+  }
+  *handle = dt;
+  return RTMC_OK;
+}
+
+extern "C" int rtmc_free_material(void* handle) {
+  clear_error();
+  if (!handle) return RTMC_OK;
+  DeviceTables* dt = (DeviceTables*)handle;
+  for (void* q : dt->ptrs) RTMC_HIP(hipFree(q));
+  delete dt;
+  return RTMC_OK;
+}

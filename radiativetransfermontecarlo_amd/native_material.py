@@ -1,0 +1,98 @@
+"""Native material preparation (rtmc_prepare_material): MaterialClass.prepareSigma
+computed in librtmc (C++ on the host), for isotropic random media.
+
+``params_from(material, d)`` maps a MaterialClass to ``rtmc_material_params``;
+``prepare_host`` returns Sigma, Sigma', the mean free times, P2P / S2S and the
+inverse-CDF tables as numpy arrays; ``NativeMaterial`` uploads them for the
+kernels (an ``rtmc_material`` usable wherever ``DeviceMaterial.c`` is).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from .material import MaterialClass
+
+__all__ = ["params_from", "prepare_host", "NativeMaterial"]
+
+
+def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n_theta: int = 0):
+    """rtmc_material_params of an (unprepared or prepared) isotropic MaterialClass."""
+    if material.type != 'isotropic':
+        raise ValueError("native preparation covers isotropic media (calcSigmaIsotropic)")
+    law = material.SpectralLaw
+    key = {'exp': 'exp', 'power_law': 'power_law', 'gaussian': 'gaussian', 'triangular': 'triangular',
+           'low_pass': 'low_pass', 'VonKarman': 'VonKarman'}.get(law)
+    if key is None:
+        # prepared materials carry the law's display name (MaterialClass.m:479-710)
+        key = {'Exp': 'exp', 'Gauss': 'gaussian', 'Triangular': 'triangular'}.get(law)
+    if key is None:
+        raise ValueError(f"SpectralLaw {law!r} has no native preparation")
+    p = _abi.MaterialParams()
+    p.dimension = int(d)
+    p.acoustics = 1 if material.acoustics else 0
+    if material.acoustics:
+        p.v[0] = float(np.asarray(material.v).ravel()[0])
+    else:
+        p.v[0] = float(material.vp)
+        p.v[1] = float(material.vs)
+    p.frequency = float(np.asarray(material.Frequency).ravel()[0])
+    cv = np.asarray(material.coefficients_of_variation, dtype=np.float64).ravel()
+    cc = np.asarray(material.correlation_coefficients, dtype=np.float64).ravel()
+    for k in range(min(3, cv.size)):
+        p.cv[k] = cv[k]
+    for k in range(min(3, cc.size)):
+        p.corr[k] = cc[k]
+    p.spectral_law = _abi.PSDF[key]
+    p.lc = float(material.CorrelationLength)
+    p.nu = float((material.SpectralParam or {}).get('nu', 0.0))
+    Q = np.asarray(material.Q if material.Q is not None else math.inf, dtype=np.float64).ravel()
+    p.Q[0] = Q[0]
+    p.Q[1] = Q[1] if Q.size > 1 else Q[0]
+    p.knot_threshold = float(knot_threshold)
+    p.n_theta = int(n_theta)
+    return p
+
+
+def prepare_host(params) -> dict:
+    """rtmc_prepare_material_host as numpy arrays."""
+    lib = _abi.load()
+    h = _abi.MaterialHost()
+    _abi.check(lib.rtmc_prepare_material_host(C.byref(params), C.byref(h)), "rtmc_prepare_material_host")
+    try:
+        nch = 1 if params.acoustics else 4
+        tables = []
+        for c in range(nch):
+            n = int(h.n_knots[c])
+            tables.append((np.ctypeslib.as_array(h.x[c], shape=(n,)).copy(),
+                           np.ctypeslib.as_array(h.y[c], shape=(n,)).copy()))
+        return {"Sigma": np.array(h.Sigma[:nch]), "Sigma_prime": np.array(h.Sigma_prime[:nch]),
+                "mft": np.array(h.mft[:]), "P2P": h.P2P, "S2S": h.S2S, "tables": tables,
+                "extrap": "nearest" if h.extrap == _abi.EXTRAP_NEAREST else "linear"}
+    finally:
+        lib.rtmc_free_material_host(C.byref(h))
+
+
+class NativeMaterial:
+    """rtmc_prepare_material: the tables in device memory, freed with the object."""
+
+    def __init__(self, params):
+        self._lib = _abi.load()
+        self.c = _abi.Material()
+        self._h = C.c_void_p()
+        _abi.check(self._lib.rtmc_prepare_material(C.byref(params), C.byref(self.c), C.byref(self._h)),
+                   "rtmc_prepare_material")
+
+    def close(self):
+        if self._h:
+            self._lib.rtmc_free_material(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,81 @@
+"""rtmc_prepare_material_host against the host MaterialClass.prepareSigma
+(both restate MaterialClass.m:172-359, 381-710, 2379-2463): Sigma to 1e-10,
+mean free times and P2P/S2S, and the inverse-CDF tables (the same knots, cdf
+values to 1e-11) for every spectral law and both media."""
+import math
+
+import numpy as np
+import pytest
+
+from radiativetransfermontecarlo_amd import native_material as nm
+from radiativetransfermontecarlo_amd.material import MaterialClass
+from tests import cases
+
+
+def _acoustic(d, law, lc=0.2, nu=None, f=10.0, cv=(0.05, 0.05), corr=0.0):
+    g = {'dimension': d}
+    m = MaterialClass(g, f, True, 1, list(cv), corr, law, lc)
+    if nu is not None:
+        m.SpectralParam = {'nu': nu}
+    return m
+
+
+CASES = [
+    ("acoustic 3-D exp (README case)", lambda: cases.config1()[2], 3),
+    ("acoustic 2-D exp", lambda: _acoustic(2, 'exp'), 2),
+    ("acoustic 3-D gaussian", lambda: _acoustic(3, 'gaussian', lc=0.05), 3),
+    ("acoustic 2-D gaussian, correlated", lambda: _acoustic(2, 'gaussian', corr=0.5, cv=(0.1, 0.03)), 2),
+    ("acoustic 3-D power law", lambda: _acoustic(3, 'power_law'), 3),
+    ("acoustic 2-D power law (Bessel K1)", lambda: _acoustic(2, 'power_law', lc=0.3), 2),
+    ("acoustic 3-D triangular", lambda: _acoustic(3, 'triangular', lc=0.3), 3),
+    ("acoustic 3-D low pass", lambda: _acoustic(3, 'low_pass', lc=0.3), 3),
+    ("acoustic 3-D von Karman", lambda: _acoustic(3, 'VonKarman', nu=0.5), 3),
+    ("acoustic 2-D von Karman", lambda: _acoustic(2, 'VonKarman', nu=0.3), 2),
+    ("elastic 2-D gaussian (2-D elastic workload)", lambda: cases.config2()[2], 2),
+    ("elastic 3-D von Karman (half-space workload)", lambda: cases.config3()[2], 3),
+]
+
+
+def _compare(material, d, knot_threshold=1e-8):
+    params = nm.params_from(material, d, knot_threshold=knot_threshold)
+    got = nm.prepare_host(params)
+    ref = MaterialClass.prepareSigma(material, d, knot_threshold=knot_threshold)
+    if ref.acoustics:
+        S_ref = [ref.Sigma]
+        inv = [ref.invcdf]
+        mft = [ref.meanFreeTime, ref.meanFreeTime]
+    else:
+        S_ref = list(np.asarray(ref.Sigma).ravel())
+        inv = [ref.invcdf[0][0], ref.invcdf[0][1], ref.invcdf[1][0], ref.invcdf[1][1]]
+        mft = list(np.asarray(ref.meanFreeTime).ravel())
+        assert got["P2P"] == pytest.approx(ref.P2P, rel=1e-10)
+        assert got["S2S"] == pytest.approx(ref.S2S, rel=1e-10)
+    assert np.allclose(got["Sigma"], S_ref, rtol=1e-10, atol=0)
+    assert np.allclose(got["mft"], mft, rtol=1e-10, atol=0)
+    for (x, y), t in zip(got["tables"], inv):
+        assert x.size == t.x.size, (x.size, t.x.size)
+        assert np.array_equal(y, t.y)
+        assert np.max(np.abs(x - t.x)) < 1e-11   # cdf values in [0, 1]; Sigma (their scale) agrees to ~1e-12
+        assert got["extrap"] == t.extrap
+    return got
+
+
+@pytest.mark.parametrize("name,build,d", CASES, ids=[c[0] for c in CASES])
+def test_native_preparation_matches_prepare_sigma(name, build, d):
+    _compare(build(), d)
+
+
+def test_native_full_tables():
+    """knot_threshold = 0: every increasing knot, nearest extrapolation."""
+    got = _compare(_acoustic(3, 'exp', lc=0.2), 3, knot_threshold=0.0)
+    assert got["extrap"] == "nearest" and got["tables"][0][0].size > 900_000
+
+
+def test_native_rejects_bad_input():
+    m = _acoustic(2, 'triangular')
+    with pytest.raises(Exception, match="dimensions other than 3"):
+        nm.prepare_host(nm.params_from(m, 2))
+    p = nm.params_from(_acoustic(3, 'exp'), 3)
+    p.corr[0] = 1.5
+    with pytest.raises(Exception, match="less than 1"):
+        nm.prepare_host(p)
