@@ -538,7 +538,7 @@ __device__ __forceinline__ int bin_guess2(const double* __restrict__ e, int ne, 
   const double g = (v - e0) * inv_w;
   int k = (g >= 0.0 && g < (double)nb) ? (int)g : (g >= (double)nb ? nb - 1 : 0);
   const double lo = e[k], hi = e[k + 1];
-  if (v >= lo && (v < hi || k == nb - 1)) return k;
+  if ((v >= lo) & ((v < hi) | (k == nb - 1))) return k;  // both edges read unconditionally (one LDS access)
   while (k > 0 && v < e[k]) --k;
   while (k < nb - 1 && v >= e[k + 1]) ++k;
   return k;

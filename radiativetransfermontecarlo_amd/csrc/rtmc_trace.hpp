@@ -242,6 +242,12 @@ struct BinSetup {
   double e0[4];
   double ehi[4];   // last edge
   double invw[4];
+  // radius the only binned position variable, edges >= 0 and finite: the bin is
+  // found from the squared radius against squared edges (held in LDS in place of
+  // the edges), with an FP32 square root for the first guess
+  unsigned radial;
+  float e0f, invwf;
+  double s_lo, s_hi;  // squared first and last edges
 };
 
 // Is a one-bin variable's range check always passed?  (then it is skipped)
@@ -273,6 +279,16 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
     double w = hi - lo;
     bs->invw[v] = (isfinite(w) && w > 0.0) ? (double)(p.ne[v] - 1) / w : 0.0;
   }
+  const bool polar = dim == 2 ? p.G.frame != RTMC_FRAME_CARTESIAN
+                              : (p.G.frame == RTMC_FRAME_SPHERICAL || p.G.frame == RTMC_FRAME_CYLINDRICAL);
+  bs->radial = (sedges != nullptr && polar && (bs->check & 7u) == 1u && bs->e0[0] >= 0.0 && isfinite(bs->ehi[0]) &&
+                bs->invw[0] > 0.0)
+                   ? 1u
+                   : 0u;
+  bs->e0f = (float)bs->e0[0];
+  bs->invwf = (float)bs->invw[0];
+  bs->s_lo = bs->e0[0] * bs->e0[0];
+  bs->s_hi = bs->ehi[0] * bs->ehi[0];
 }
 
 // Birth (initializeParticle.m:17-147), computed in FP64 for both flight
@@ -388,13 +404,29 @@ __device__ __forceinline__ int dir_bin(const LaunchParams& p, const BinSetup& bs
 template <typename R>
 __device__ __forceinline__ int bin_index(const LaunchParams& p, const BinSetup& bs,
                                          const double* __restrict__ sedges, int dim, const R X[3], int dbin) {
-  double v[4];
-  obs_vars<R, true>(dim, p.G.frame, X, R(0), bs.check & 7u, v);
   uint32_t idx = 0;
   if (bs.check & 8u) {
     if (dbin < 0) return -1;
     idx = (uint32_t)dbin * p.stride[3];
   }
+  if (bs.radial) {
+    // e_k <= r < e_k+1  <=>  e_k^2 <= r^2 < e_k+1^2 for edges >= 0 (last bin closed)
+    const double x = X[0], y = X[1], z = X[2];
+    const double s = (dim == 3 && p.G.frame == RTMC_FRAME_SPHERICAL) ? x * x + y * y + z * z : x * x + y * y;
+    if (!(s >= bs.s_lo && s <= bs.s_hi)) return -1;
+    const int nb = p.ne[0] - 1;
+    const float g = (__builtin_amdgcn_sqrtf((float)s) - bs.e0f) * bs.invwf;  // guess only: v_sqrt_f32 as is
+    int k = (int)fminf(fmaxf(g, 0.0f), (float)(nb - 1));  // NaN -> 0
+    const double* e2 = sedges + p.eoff[0];
+    const double lo = e2[k], hi = e2[k + 1];
+    if (!((s >= lo) & ((s < hi) | (k == nb - 1)))) {
+      while (k > 0 && s < e2[k]) --k;
+      while (k < nb - 1 && s >= e2[k + 1]) ++k;
+    }
+    return (int)(idx + (uint32_t)k * p.stride[0]);
+  }
+  double v[4];
+  obs_vars<R, true>(dim, p.G.frame, X, R(0), bs.check & 7u, v);
   if (p.edges_total > 0) {
     // edges in LDS (`sedges` points into the work-group's LDS block)
 #pragma unroll
@@ -478,9 +510,17 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) bin_setup(p, DIM, sedges, &s_bs);
+  if (threadIdx.x == 0) bin_setup(p, DIM, p.edges_total > 0 ? sedges : nullptr, &s_bs);
   __syncthreads();
-  const BinSetup bs = s_bs;
+  if (s_bs.radial) {
+    // squared radial edges for bin_index
+    for (int k = threadIdx.x; k < p.ne[0]; k += blockDim.x) {
+      const double e = smem[p.eoff[0] + k];
+      smem[p.eoff[0] + k] = e * e;
+    }
+    __syncthreads();
+  }
+  const BinSetup& bs = s_bs;  // read from LDS where used (wave-uniform values, no registers held)
   const int lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 

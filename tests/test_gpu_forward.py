@@ -234,3 +234,33 @@ def test_fp32_flight_per_bin_error_against_fp64(gpu):
     se = np.sqrt(2.0 / per_t64[sel])
     print(f"fp32 vs fp64: {st}; per-time energy max rel diff {rel.max():.2e} (se {se.max():.2e})")
     assert np.all(rel < 5 * se + 1e-6)
+
+
+@pytest.mark.parametrize("frame,d", [("spherical", 3), ("cylindrical", 3), ("spherical", 2)])
+def test_radial_bins_fine_global_edges_match_coarse(gpu, frame, d):
+    """Radius-only binning: 5000 radial bins (edges too many for LDS, read from
+    global memory, bin from the radius) summed 50 at a time equal 100 bins (edges
+    in LDS, bin from the squared radius) of the same trajectories (same seed),
+    up to positions within rounding of an edge."""
+    geometry = {'dimension': d, 'frame': frame}
+    source = {'numberParticles': 1, 'type': 'point', 'position': [0.0] * d, 'lambda': 0.05}
+    material = MaterialClass.prepareSigma(cases.iso_acoustic(d), d)
+    t = np.arange(0, 1.0 + 1e-9, 0.05)
+    n = 200_000
+
+    def hist(nx):
+        obs = {'x': np.linspace(0, 3, nx + 1), 'time': t}
+        if d == 3 and frame == 'cylindrical':
+            obs['z'] = [-np.inf, np.inf]
+        O = initializeObservation(geometry, True, obs, n)
+        E, _, _ = solver.forward_histogram(geometry, source, material, O, n, seed=21, device='cuda')
+        return O, E.cpu().numpy().view(np.uint32).astype(np.int64)
+
+    Of, Ef = hist(5000)
+    Oc, Ec = hist(100)
+    fine = Ef.reshape(Of.shape, order='F')
+    coarse = Ec.reshape(Oc.shape, order='F')
+    assert fine.shape[0] == 5000 and coarse.shape[0] == 100
+    summed = fine.reshape(100, 50, *fine.shape[1:]).sum(axis=1)
+    assert summed.sum() == coarse.sum() > 0
+    assert np.abs(summed - coarse).sum() <= 2e-5 * coarse.sum()
