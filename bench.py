@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="config 4: multiply every material's particle count (1 = the configuration's own)")
     ap.add_argument("--autotune", action="store_true", help="config 4: tune the batched launch first")
+    ap.add_argument("--mask-counts", type=int, default=100,
+                    help="config 4: bins enter the loss with at least this many counts (and a non-zero target)")
+    ap.add_argument("--dump-gradients", default="", metavar="NPZ",
+                    help="config 4: write the per-material LR and FD gradients to this .npz")
     ap.add_argument("--no-batch", action="store_true",
                     help="config 2: skip the 64-material batched forward+adjoint comparison (config 4)")
     ap.add_argument("--particles", type=int, default=0, help="0 = the configuration's own count")
@@ -288,15 +292,15 @@ def run_batch_adjoint(args, emit=True):
 
     # target: perturbed parameters, independent particles (not timed)
     pt = params * np.array([1.05, 0.95, 1.10])
-    Bt = rb.Batch(geometry, O, [rb.prepare_material(base, 3, *p, knot_threshold=KT) for p in pt], source,
-                  list(Ns), [7_000 + i for i in range(n)], device=dev)
+    Bt = rb.Batch.from_params(geometry, O, base, pt, source, list(Ns), [7_000 + i for i in range(n)], device=dev,
+                              knot_threshold=KT)
     Bt.forward()
     target = Bt.energy_density()
 
     def loss_and_weights(E):
         """loss per item and dL/dcount (the backward of the loss through the normalisation)"""
         dens = E.to(torch.float64) * scales
-        mask = (target > 0) & (E >= 16)
+        mask = (target > 0) & (E >= args.mask_counts)
         lg = torch.where(mask, torch.log(torch.where(mask, dens, 1.0)) - torch.log(torch.where(mask, target, 1.0)),
                          0.0)
         return (lg * lg).sum(dim=1), torch.where(mask, 2.0 * lg / torch.where(mask, dens, 1.0), 0.0) * scales, mask
@@ -309,8 +313,7 @@ def run_batch_adjoint(args, emit=True):
     # ---- batched forward + adjoint -------------------------------------
     sync()
     t0 = time.perf_counter()
-    mats = [rb.prepare_material(base, 3, *p, knot_threshold=KT) for p in params]
-    B = rb.Batch(geometry, O, mats, source, list(Ns), seeds, device=dev)
+    B = rb.Batch.from_params(geometry, O, base, params, source, list(Ns), seeds, device=dev, knot_threshold=KT)
     tabs = B.derivative_tables()
     t_prep_a = time.perf_counter() - t0
     tuning = B.autotune(0.02) if (args.autotune or not emit) else None
@@ -341,6 +344,27 @@ def run_batch_adjoint(args, emit=True):
     mask_np = mask.cpu().numpy()
 
     # ---- plain forward solves + central differences ---------------------
+    # one material at a time with the faster single-material preparation here:
+    # the Python MaterialClass.prepareSigma or rtmc_prepare_material (threaded grid)
+    from radiativetransfermontecarlo_amd import native_material as nm
+
+    def prep_python(p):
+        m = rb.prepare_material(base, 3, *p, knot_threshold=KT)
+        return m, solver.DeviceMaterial(m, 3, dev)
+
+    def prep_native(p):
+        m = nm.NativeMaterial(nm.params_for(base, 3, *p, knot_threshold=KT))
+        return base, m
+
+    t_py = t_nat = 0.0
+    for p in params[:3]:
+        tp = time.perf_counter()
+        prep_python(p)
+        t_py += time.perf_counter() - tp
+        tp = time.perf_counter()
+        prep_native(p)
+        t_nat += time.perf_counter() - tp
+    prep_one, prep_kind = (prep_native, "rtmc_prepare_material") if t_nat < t_py else (prep_python, "python")
     t_prep_b = 0.0
     gpu_ms_b = 0.0
     dbins = solver.DeviceBins(O, dev)
@@ -353,13 +377,13 @@ def run_batch_adjoint(args, emit=True):
         msk = mask[i]
         sc = scales[i]
 
-        def solve(p, m=None):
+        def solve(p, md=None):
             nonlocal t_prep_b, gpu_ms_b
-            if m is None:
+            if md is None:
                 tp = time.perf_counter()
-                m = rb.prepare_material(base, 3, *p, knot_threshold=KT)
+                md = prep_one(p)
                 t_prep_b += time.perf_counter() - tp
-            dm = solver.DeviceMaterial(m, 3, dev)
+            m, dm = md
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
             E, _c, _k = solver.forward_histogram(geometry, source, m, O, int(Ns[i]), seed=seeds[i], device=dev,
@@ -372,7 +396,7 @@ def run_batch_adjoint(args, emit=True):
             return L
 
         tp = time.perf_counter()
-        m0 = rb.prepare_material(base, 3, *params[i], knot_threshold=KT)
+        m0 = prep_one(params[i])
         t_prep_b += time.perf_counter() - tp
         solve(params[i], m0)
         for k in range(3):
@@ -386,6 +410,8 @@ def run_batch_adjoint(args, emit=True):
     _abi.check(lib.rtmc_set_trace_timing(0), "rtmc_set_trace_timing")
     kernel_ms_b = float(sum(tkb[i] for i in range(nk.value)))
 
+    if args.dump_gradients:
+        np.savez(args.dump_gradients, lr=g_lr, fd=g_fd, params=params, Ns=Ns)
     rel = np.abs(g_lr - g_fd) / np.maximum(np.abs(g_fd), 1e-300)
     agree = {name: {"median_rel_diff": float(np.median(rel[:, k])), "within_2pct": int(np.sum(rel[:, k] < 0.02)),
                     "within_10pct": int(np.sum(rel[:, k] < 0.10)),
@@ -404,15 +430,18 @@ def run_batch_adjoint(args, emit=True):
         "config": {"workload": "64 3-D acoustic materials, cv=[c c] c~U[.02,.15], lc~U[.05,.5], Q~U[50,1000], "
                                "exp ACF, f=10, README geometry/observation; particles log-uniform 1e5..1e7 "
                                "(config 4)", "particles_total": int(Ns.sum()), "batch_scale": args.batch_scale,
-                   "loss": "sum over bins with E_target>0 and >=16 counts of (log E - log E_target)^2; target at "
-                           "(+5% c, -5% lc, +10% Q), independent seed",
+                   "loss": f"sum over bins with E_target>0 and >={args.mask_counts} counts of (log E - log E_target)^2; "
+                          "target at (+5% c, -5% lc, +10% Q), independent seed",
                    "fd_steps_rel": H.tolist(), "tables": "knot_threshold=0"},
-        "batched": {"wall_s": total_a, "prep_s": t_prep_a, "solve_s": t_a, "forward_ms": ms_fwd,
+        "batched": {"wall_s": total_a, "prep_s": t_prep_a,
+                    "preparation": "rtmc_prepare_materials + rtmc_lc_derivatives (all items at once)",
+                    "solve_s": t_a, "forward_ms": ms_fwd,
                     "adjoint_ms": ms_adj, "forward_kernel_ms": k_fwd, "adjoint_kernel_ms": k_adj,
                     "scatter_events": nscat, "deposits": ndep,
                     "events_per_s_forward": nscat / (ms_fwd * 1e-3), "tuning": tuning},
         "plain_fd": {"wall_s": total_b, "prep_s": t_prep_b, "solves": 7 * n, "gpu_ms": gpu_ms_b,
-                     "kernel_ms": kernel_ms_b},
+                     "kernel_ms": kernel_ms_b, "preparation": prep_kind,
+                     "prep_probe_s": {"python": t_py / 3, "rtmc_prepare_material": t_nat / 3}},
         "gpu_only_speedup": gpu_ms_b / (ms_fwd + ms_adj),
         "kernel_only_speedup": kernel_ms_b / (k_fwd + k_adj),
         "prep_excluded_speedup": (total_b - t_prep_b) / t_a,

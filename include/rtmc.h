@@ -433,6 +433,22 @@ void rtmc_free_material_host(rtmc_material_host* out);
 int rtmc_prepare_material(const rtmc_material_params* params, rtmc_material* out, void** handle);
 int rtmc_free_material(void* handle);
 
+/* No counterpart in the reference (it prepares one material per call): n
+ * materials prepared at once for the batch entry points, one material per
+ * host thread, all tables in one device block owned by *handle
+ * (rtmc_free_material).  Errors name the first failing material. */
+int rtmc_prepare_materials(const rtmc_material_params* params, int32_t n, rtmc_material* out,
+                           void** handle);
+
+/* No counterpart in the reference: the material derivatives the adjoint
+ * (rtmc_adjoint_batched) needs, by central differences of calcSigmaIsotropic
+ * (MaterialClass.m:204-235) at lc (1 +- rel), acoustic media:
+ * dlogsigma_dlc[i * n_theta + k] = d log sigma(theta_k) / d lc on
+ * theta = linspace(0, pi, n_theta) (0 where sigma is not positive), and
+ * dlogSigma_dlc[i] = d log Sigma / d lc. */
+int rtmc_lc_derivatives(const rtmc_material_params* params, int32_t n, int64_t n_theta, double rel,
+                        double* dlogsigma_dlc, double* dlogSigma_dlc);
+
 /* Measurement aid, no counterpart in the reference: throughput of no-return
  * global atomics (requests/s) on the current device, the ceiling of the
  * fused kernel's deposits when the histogram is in global memory.  buf:

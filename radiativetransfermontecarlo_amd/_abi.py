@@ -291,6 +291,8 @@ EXPORTS = (
     "rtmc_free_material_host",
     "rtmc_prepare_material",
     "rtmc_free_material",
+    "rtmc_prepare_materials",
+    "rtmc_lc_derivatives",
     "rtmc_probe_atomics",
     "rtmc_set_trace_timing",
     "rtmc_trace_times",
@@ -335,6 +337,11 @@ def _declare(lib):
     lib.rtmc_prepare_material.argtypes = [P(MaterialParams), P(Material), P(C.c_void_p)]
     lib.rtmc_free_material.restype = C.c_int
     lib.rtmc_free_material.argtypes = [C.c_void_p]
+    lib.rtmc_prepare_materials.restype = C.c_int
+    lib.rtmc_prepare_materials.argtypes = [P(MaterialParams), C.c_int32, P(Material), P(C.c_void_p)]
+    lib.rtmc_lc_derivatives.restype = C.c_int
+    lib.rtmc_lc_derivatives.argtypes = [P(MaterialParams), C.c_int32, C.c_int64, C.c_double, P(C.c_double),
+                                        P(C.c_double)]
     lib.rtmc_set_trace_timing.restype = C.c_int
     lib.rtmc_set_trace_timing.argtypes = [C.c_int32]
     lib.rtmc_trace_times.restype = C.c_int

@@ -122,8 +122,12 @@ class Batch:
         self.keep = []
         self.geometry = build_geometry(geometry)
         self.bins = DeviceBins(O, self.dev)
+        # prepared MaterialClass objects are uploaded here; anything else carries
+        # its rtmc_material in `.c` (native_material.NativeMaterials.items)
         self.dmats = [DeviceMaterial(m, self.d, self.dev, with_zoeppritz=self.geometry.n_bnd > 0)
-                      for m in materials]
+                      if isinstance(m, MaterialClass) else m for m in materials]
+        self._tables = None
+        self._native = None
         self.E = torch.zeros((n, O.size), dtype=torch.int32, device=self.dev)
         self.counters = torch.zeros((n, _abi.NCOUNTERS), dtype=torch.int64, device=self.dev)
         self.items = (_abi.BatchItem * max(n, 1))()
@@ -178,8 +182,27 @@ class Batch:
             self.configure(t.block_size, t.particles_per_thread, t.hist_mode, t.event_batch)
         return out
 
+    @classmethod
+    def from_params(cls, geometry, O: Observation, base: MaterialClass, params, sources, Ns: Sequence[int],
+                    seeds: Sequence[int], *, knot_threshold: float = 0.0, n_glc: int = 2049, **kw) -> "Batch":
+        """A batch of ``base`` at the (c, lc, Q) rows of ``params``, its materials
+        and the adjoint's derivative tables prepared natively for all items at
+        once (rtmc_prepare_materials, rtmc_lc_derivatives)."""
+        from . import native_material as nm
+        d = int(_get(geometry, 'dimension'))
+        plist = [nm.params_for(base, d, c, lc, Q, knot_threshold=knot_threshold) for c, lc, Q in params]
+        nat = nm.NativeMaterials(plist)
+        B = cls(geometry, O, nat.items, sources, Ns, seeds, **kw)
+        B._native = nat  # owns the device tables
+        if base.acoustics:
+            g, dS = nm.lc_derivatives(plist, n_glc)
+            B._tables = [(float(p.cv[0]), float(dS[i]), g[i]) for i, p in enumerate(plist)]
+        return B
+
     def derivative_tables(self, n: int = 2049):
         """(c, dlogSigma_dlc, dlogsigma_dlc table) of every item."""
+        if self._tables is not None and all(np.size(t[2]) == n for t in self._tables):
+            return self._tables
         out = []
         for m in self.materials:
             g, dS = lc_derivatives(m, self.d, n)

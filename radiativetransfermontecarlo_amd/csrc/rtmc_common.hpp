@@ -17,6 +17,7 @@ namespace rtmc {
 
 void set_error(const char* fmt, ...);
 void clear_error();
+const char* last_error();
 
 // Returns RTMC_ERR_HIP (and records the message) when a HIP call fails.
 #define RTMC_HIP(call)                                                                  \

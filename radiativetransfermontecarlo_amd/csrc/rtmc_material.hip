@@ -6,9 +6,11 @@
 #include "rtmc_common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -203,8 +205,8 @@ double integral(const F& f, double a, double b) {
 
 // prepareSigmaOne (MaterialClass.m:2443-2463): Sigma, Sigma' and the inverse
 // CDF of the scattering angle on n points of [0, pi].
-void prepare_one(const Sigma& sg, int d, int64_t n, double thr, double* S, double* Sp, std::vector<double>* kx,
-                 std::vector<double>* ky) {
+void prepare_one(const Sigma& sg, int d, int64_t n, double thr, unsigned threads, double* S, double* Sp,
+                 std::vector<double>* kx, std::vector<double>* ky) {
   if (d == 2) {
     *S = 2.0 * integral(sg, 0.0, kPiH);
     *Sp = 2.0 * integral([&](double t) { return sg(t) * std::cos(t); }, 0.0, kPiH);
@@ -217,19 +219,22 @@ void prepare_one(const Sigma& sg, int d, int64_t n, double thr, double* S, doubl
   // pdf on the grid in parallel chunks, then the running sum in order (nancumsum)
   {
     const double Sv = *S;
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
+    const unsigned nt = std::max(1u, threads);
     const int64_t per = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-      th.emplace_back([&, t]() {
-        const int64_t i0 = (int64_t)t * per, i1 = std::min(n, i0 + per);
-        for (int64_t i = i0; i < i1; ++i) {
-          const double x = (i == n - 1) ? kPiH : (double)i * step;
-          cdf[(size_t)i] = (d == 2) ? (2.0 / Sv) * sg(x) : (2.0 * kPiH / Sv) * std::sin(x) * sg(x);
-        }
-      });
+    auto part = [&](unsigned t) {
+      const int64_t i0 = (int64_t)t * per, i1 = std::min(n, i0 + per);
+      for (int64_t i = i0; i < i1; ++i) {
+        const double x = (i == n - 1) ? kPiH : (double)i * step;
+        cdf[(size_t)i] = (d == 2) ? (2.0 / Sv) * sg(x) : (2.0 * kPiH / Sv) * std::sin(x) * sg(x);
+      }
+    };
+    if (nt == 1) {
+      part(0);
+    } else {
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nt; ++t) th.emplace_back(part, t);
+      for (auto& t : th) t.join();
     }
-    for (auto& t : th) t.join();
   }
   double run = 0.0;
   const double dx = kPiH / (double)n;
@@ -261,21 +266,14 @@ struct DeviceTables {
 
 using namespace rtmc;
 
-extern "C" int rtmc_prepare_material_host(const rtmc_material_params* p, rtmc_material_host* out) {
-  clear_error();
-  RTMC_REQUIRE(p != nullptr && out != nullptr, "params and out must be given");
-  memset(out, 0, sizeof(*out));
-  RTMC_REQUIRE(p->dimension == 2 || p->dimension == 3, "dimension must be 2 or 3 (sigma for 1D not implemented)");
-  RTMC_REQUIRE(p->frequency > 0.0 && p->lc > 0.0, "frequency and correlation length must be > 0");
-  RTMC_REQUIRE(p->acoustics ? p->v[0] > 0.0 : (p->v[0] > 0.0 && p->v[1] > 0.0), "velocities must be > 0");
-  if (p->acoustics) {
-    RTMC_REQUIRE(std::fabs(p->corr[0]) <= 1.0, "Absolute value of corr(kappa,rho) should be less than 1!");
-  } else {
-    for (int k = 0; k < 3; ++k)
-      RTMC_REQUIRE(std::fabs(p->corr[k]) <= 1.0, "Absolute values of correlation coefficients should be less than 1!");
-  }
-  const int64_t n = p->n_theta > 0 ? p->n_theta : 1000000;
-  RTMC_REQUIRE(n >= 16, "n_theta must be >= 16");
+namespace rtmc {
+namespace {
+
+unsigned host_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+// The material's cross-section functor for channel 0 (acoustic) with the
+// frequency, velocities and fluctuation statistics of `p` (calcSigmaIsotropic).
+int make_sigma(const rtmc_material_params* p, Sigma* out) {
   Psdf phi;
   int e = make_psdf(*p, &phi);
   if (e) return e;
@@ -301,18 +299,48 @@ extern "C" int rtmc_prepare_material_host(const rtmc_material_params* p, rtmc_ma
     base.rlr = p->corr[1];
     base.rmr = p->corr[2];
   }
+  *out = base;
+  return RTMC_OK;
+}
+
+int check_params(const rtmc_material_params* p) {
+  RTMC_REQUIRE(p->dimension == 2 || p->dimension == 3, "dimension must be 2 or 3 (sigma for 1D not implemented)");
+  RTMC_REQUIRE(p->frequency > 0.0 && p->lc > 0.0, "frequency and correlation length must be > 0");
+  RTMC_REQUIRE(p->acoustics ? p->v[0] > 0.0 : (p->v[0] > 0.0 && p->v[1] > 0.0), "velocities must be > 0");
+  if (p->acoustics) {
+    RTMC_REQUIRE(std::fabs(p->corr[0]) <= 1.0, "Absolute value of corr(kappa,rho) should be less than 1!");
+  } else {
+    for (int k = 0; k < 3; ++k)
+      RTMC_REQUIRE(std::fabs(p->corr[k]) <= 1.0, "Absolute values of correlation coefficients should be less than 1!");
+  }
+  RTMC_REQUIRE(p->n_theta == 0 || p->n_theta >= 16, "n_theta must be >= 16");
+  return RTMC_OK;
+}
+
+// rtmc_prepare_material_host with `threads` host threads (1: all on the caller's)
+int prepare_host(const rtmc_material_params* p, rtmc_material_host* out, unsigned threads) {
+  RTMC_REQUIRE(p != nullptr && out != nullptr, "params and out must be given");
+  memset(out, 0, sizeof(*out));
+  int e = check_params(p);
+  if (e) return e;
+  const int64_t n = p->n_theta > 0 ? p->n_theta : 1000000;
+  Sigma base;
+  e = make_sigma(p, &base);
+  if (e) return e;
   const int nch = p->acoustics ? 1 : 4;
   std::vector<double> kx[4], ky[4];
   double S[4] = {0, 0, 0, 0}, Sp[4] = {0, 0, 0, 0};
-  {
+  auto channel = [&](int c, unsigned t) {
+    Sigma sg = base;
+    sg.channel = c;
+    prepare_one(sg, p->dimension, n, p->knot_threshold, t, &S[c], &Sp[c], &kx[c], &ky[c]);
+  };
+  if (threads <= 1 || nch == 1) {
+    for (int c = 0; c < nch; ++c) channel(c, threads);
+  } else {
     std::vector<std::thread> th;
-    for (int c = 0; c < nch; ++c) {
-      th.emplace_back([&, c]() {
-        Sigma sg = base;
-        sg.channel = c;
-        prepare_one(sg, p->dimension, n, p->knot_threshold, &S[c], &Sp[c], &kx[c], &ky[c]);
-      });
-    }
+    const unsigned per = std::max(1u, threads / (unsigned)nch);
+    for (int c = 0; c < nch; ++c) th.emplace_back(channel, c, per);
     for (auto& t : th) t.join();
   }
   for (int c = 0; c < nch; ++c) {
@@ -344,6 +372,50 @@ extern "C" int rtmc_prepare_material_host(const rtmc_material_params* p, rtmc_ma
     out->S2S = S[3] / (S[3] + S[2]);
   }
   return RTMC_OK;
+}
+
+// Runs f(i) for i in [0, n) on up to host_threads() workers; returns the
+// status of the first failure (its message re-raised on the caller's thread).
+template <class F>
+int parallel_items(int n, F f) {
+  const unsigned nw = std::max(1u, std::min(host_threads(), (unsigned)std::max(n, 1)));
+  std::atomic<int> next(0);
+  std::atomic<int> bad(-1);
+  std::vector<int> rc((size_t)std::max(n, 1), RTMC_OK);
+  std::vector<std::string> msg((size_t)std::max(n, 1));
+  auto work = [&]() {
+    for (int i = next++; i < n; i = next++) {
+      if (bad.load() >= 0) break;
+      clear_error();
+      rc[(size_t)i] = f(i);
+      if (rc[(size_t)i] != RTMC_OK) {
+        msg[(size_t)i] = last_error();
+        int expect = -1;
+        bad.compare_exchange_strong(expect, i);
+      }
+    }
+  };
+  if (nw == 1) {
+    work();
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nw; ++t) th.emplace_back(work);
+    for (auto& t : th) t.join();
+  }
+  const int b = bad.load();
+  if (b >= 0) {
+    set_error("material %d: %s", b, msg[(size_t)b].c_str());
+    return rc[(size_t)b];
+  }
+  return RTMC_OK;
+}
+
+}  // namespace
+}  // namespace rtmc
+
+extern "C" int rtmc_prepare_material_host(const rtmc_material_params* p, rtmc_material_host* out) {
+  clear_error();
+  return prepare_host(p, out, host_threads());
 }
 
 extern "C" void rtmc_free_material_host(rtmc_material_host* out) {
@@ -402,6 +474,118 @@ extern "C" int rtmc_prepare_material(const rtmc_material_params* p, rtmc_materia
   }
   *handle = dt;
   return RTMC_OK;
+}
+
+extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_t n, rtmc_material* out,
+                                      void** handle) {
+  clear_error();
+  RTMC_REQUIRE(handle != nullptr, "handle must be given");
+  *handle = nullptr;
+  RTMC_REQUIRE(n >= 0 && (n == 0 || (params != nullptr && out != nullptr)), "params and out must be given");
+  if (n == 0) return RTMC_OK;
+  // host tables: one material per worker, the angle grid of each on its worker
+  std::vector<rtmc_material_host> h((size_t)n);
+  for (auto& x : h) memset(&x, 0, sizeof(x));
+  auto free_all = [&]() {
+    for (auto& x : h) rtmc_free_material_host(&x);
+  };
+  int e = parallel_items(n, [&](int i) { return prepare_host(&params[i], &h[(size_t)i], 1); });
+  if (e) {
+    free_all();
+    return e;
+  }
+  // one device block for every table: x then y of each channel, 8-byte aligned
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    const int nch = params[i].acoustics ? 1 : 4;
+    for (int c = 0; c < nch; ++c) total += 2 * (size_t)h[(size_t)i].n_knots[c];
+  }
+  std::vector<double> pack(total);
+  void* dev = nullptr;
+  if (hipMalloc(&dev, total * sizeof(double)) != hipSuccess) {
+    free_all();
+    set_error("hipMalloc of %zu bytes of material tables failed", total * sizeof(double));
+    return RTMC_ERR_NOMEM;
+  }
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    const rtmc_material_params* p = &params[i];
+    const rtmc_material_host& hi = h[(size_t)i];
+    rtmc_material& m = out[i];
+    memset(&m, 0, sizeof(m));
+    m.acoustics = p->acoustics;
+    m.vp = p->v[0];
+    m.vs = p->acoustics ? 0.0 : p->v[1];
+    m.mft[0] = hi.mft[0];
+    m.mft[1] = hi.mft[1];
+    m.P2P = hi.P2P;
+    m.S2S = hi.S2S;
+    m.QP = p->Q[0];
+    m.QS = p->acoustics ? p->Q[0] : p->Q[1];
+    m.frequency = p->frequency;
+    const int nch = p->acoustics ? 1 : 4;
+    for (int c = 0; c < nch; ++c) {
+      const size_t k = (size_t)hi.n_knots[c];
+      memcpy(pack.data() + off, hi.x[c], k * sizeof(double));
+      memcpy(pack.data() + off + k, hi.y[c], k * sizeof(double));
+      m.invcdf[c].n = (int64_t)k;
+      m.invcdf[c].x = (const double*)dev + off;
+      m.invcdf[c].y = (const double*)dev + off + k;
+      m.invcdf[c].extrap = hi.extrap;
+      off += 2 * k;
+    }
+  }
+  free_all();
+  if (hipMemcpy(dev, pack.data(), total * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dev);
+    set_error("uploading the material tables failed");
+    return RTMC_ERR_NOMEM;
+  }
+  DeviceTables* dt = new DeviceTables();
+  dt->ptrs.push_back(dev);
+  *handle = dt;
+  return RTMC_OK;
+}
+
+extern "C" int rtmc_lc_derivatives(const rtmc_material_params* params, int32_t n, int64_t n_theta, double rel,
+                                   double* dlogsigma_dlc, double* dlogSigma_dlc) {
+  clear_error();
+  RTMC_REQUIRE(n >= 0 && (n == 0 || (params != nullptr && dlogsigma_dlc != nullptr && dlogSigma_dlc != nullptr)),
+               "params and outputs must be given");
+  RTMC_REQUIRE(n_theta >= 2, "n_theta must be >= 2");
+  RTMC_REQUIRE(rel > 0.0 && rel < 0.5, "rel must be in (0, 0.5)");
+  return parallel_items(n, [&](int i) -> int {
+    const rtmc_material_params& p = params[i];
+    RTMC_REQUIRE(p.acoustics, "lc derivatives are implemented for acoustic media");
+    int e = check_params(&p);
+    if (e) return e;
+    const double h = rel * p.lc;
+    rtmc_material_params pp = p, pm = p;
+    pp.lc = p.lc + h;
+    pm.lc = p.lc - h;
+    Sigma sp, sm;
+    e = make_sigma(&pp, &sp);
+    if (e) return e;
+    e = make_sigma(&pm, &sm);
+    if (e) return e;
+    double* g = dlogsigma_dlc + (size_t)i * (size_t)n_theta;
+    for (int64_t k = 0; k < n_theta; ++k) {
+      const double th = (k == n_theta - 1) ? kPiH : (double)k * (kPiH / (double)(n_theta - 1));
+      const double a = sp(th), b = sm(th);
+      g[k] = (a > 0.0 && b > 0.0) ? (std::log(a) - std::log(b)) / (2.0 * h) : 0.0;
+    }
+    // total cross-sections as prepareSigmaOne integrates them (MaterialClass.m:2443-2455)
+    double Sp_, Sm_;
+    if (p.dimension == 2) {
+      Sp_ = 2.0 * integral(sp, 0.0, kPiH);
+      Sm_ = 2.0 * integral(sm, 0.0, kPiH);
+    } else {
+      Sp_ = 2.0 * kPiH * integral([&](double t) { return sp(t) * std::sin(t); }, 0.0, kPiH);
+      Sm_ = 2.0 * kPiH * integral([&](double t) { return sm(t) * std::sin(t); }, 0.0, kPiH);
+    }
+    dlogSigma_dlc[i] = (std::log(Sp_) - std::log(Sm_)) / (2.0 * h);
+    return RTMC_OK;
+  });
 }
 
 extern "C" int rtmc_free_material(void* handle) {

@@ -16,7 +16,7 @@ import numpy as np
 from . import _abi
 from .material import MaterialClass
 
-__all__ = ["params_from", "prepare_host", "NativeMaterial"]
+__all__ = ["params_from", "params_for", "prepare_host", "NativeMaterial", "NativeMaterials", "lc_derivatives"]
 
 
 def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n_theta: int = 0):
@@ -54,6 +54,19 @@ def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n
     p.Q[1] = Q[1] if Q.size > 1 else Q[0]
     p.knot_threshold = float(knot_threshold)
     p.n_theta = int(n_theta)
+    return p
+
+
+def params_for(base: MaterialClass, d: int, c: float, lc: float, Q: float, knot_threshold: float = 1e-8,
+               n_theta: int = 0):
+    """params_from of ``base`` with the coefficients of variation scaled to
+    cv[0] = c, the correlation length lc and the attenuation Q (acoustic)."""
+    p = params_from(base, d, knot_threshold=knot_threshold, n_theta=n_theta)
+    s = float(c) / p.cv[0]
+    for k in range(3):
+        p.cv[k] *= s
+    p.lc = float(lc)
+    p.Q[0] = p.Q[1] = float(Q)
     return p
 
 
@@ -96,3 +109,47 @@ class NativeMaterial:
             self.close()
         except Exception:
             pass
+
+
+class NativeMaterials:
+    """rtmc_prepare_materials: n materials prepared at once (one per host
+    thread), their tables in one device block freed with the object.  ``items``
+    holds one object per material whose ``.c`` is its rtmc_material."""
+
+    class _Item:
+        def __init__(self, c):
+            self.c = c
+
+    def __init__(self, params_list):
+        self._lib = _abi.load()
+        n = len(params_list)
+        arr = (_abi.MaterialParams * max(n, 1))(*params_list)
+        self._mats = (_abi.Material * max(n, 1))()
+        self._h = C.c_void_p()
+        _abi.check(self._lib.rtmc_prepare_materials(arr, n, self._mats, C.byref(self._h)), "rtmc_prepare_materials")
+        self.items = [NativeMaterials._Item(self._mats[i]) for i in range(n)]
+
+    def close(self):
+        if self._h:
+            self._lib.rtmc_free_material(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lc_derivatives(params_list, n_theta: int = 2049, rel: float = 1e-4):
+    """rtmc_lc_derivatives: (g, dlogSigma) with g[i] = d log sigma / d lc of
+    material i on linspace(0, pi, n_theta) and dlogSigma[i] = d log Sigma / d lc."""
+    lib = _abi.load()
+    n = len(params_list)
+    arr = (_abi.MaterialParams * max(n, 1))(*params_list)
+    g = np.zeros((n, n_theta))
+    dS = np.zeros(n)
+    _abi.check(lib.rtmc_lc_derivatives(arr, n, int(n_theta), float(rel),
+                                       g.ctypes.data_as(C.POINTER(C.c_double)),
+                                       dS.ctypes.data_as(C.POINTER(C.c_double))), "rtmc_lc_derivatives")
+    return g, dS

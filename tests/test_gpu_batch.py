@@ -341,3 +341,27 @@ def test_trace_timing_and_atomic_probe(gpu):
                    "probe")
         assert 1e8 < r.value < 1e13
     assert lib.rtmc_probe_atomics(C.c_void_p(buf.data_ptr()), 1 << 20, 7, 10, C.byref(r), stream) == _abi.ERR_ARG
+
+
+def test_batch_from_native_preparation_matches_host_preparation(gpu):
+    """Batch.from_params (rtmc_prepare_materials + rtmc_lc_derivatives) against
+    host-prepared materials and derivative tables: the same trajectories up to
+    table values that agree to ~1e-12, so histograms agree except for deposits
+    within rounding of a bin edge, and the adjoint's gradients to 1e-4."""
+    geometry, source, base, observation = cases.config1()
+    observation = dict(observation, time=np.arange(0, 1.0 + 1e-9, 0.02))
+    O = initializeObservation(geometry, True, observation, 1)
+    params = np.array([[c, lc, Q] for c, lc, Q in PARAMS])
+    Bn = rb.Batch.from_params(geometry, O, base, params, source, NS, SEEDS, device='cuda', knot_threshold=0.0)
+    mats = [rb.prepare_material(base, 3, c=c, lc=lc, Q=Q, knot_threshold=0.0) for c, lc, Q in PARAMS]
+    Bh = rb.Batch(geometry, O, mats, source, NS, SEEDS, device='cuda')
+    En, _ = Bn.forward()
+    Eh, _ = Bh.forward()
+    a = En.cpu().numpy().astype(np.int64)
+    b = Eh.cpu().numpy().astype(np.int64)
+    for i in range(len(NS)):
+        assert np.abs(a[i] - b[i]).sum() <= 1e-5 * b[i].sum() + 2, f"item {i}"
+    w = torch.linspace(0.5, 1.5, O.size, dtype=torch.float32, device='cuda').repeat(len(NS), 1)
+    gn = Bn.adjoint(w).cpu().numpy()
+    gh = Bh.adjoint(w).cpu().numpy()
+    np.testing.assert_allclose(gn, gh, rtol=1e-4, atol=1e-7 * np.abs(gh).max())

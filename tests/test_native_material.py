@@ -117,3 +117,67 @@ def test_native_tables_drive_the_kernel_like_host_tables(gpu, which):
     assert a.sum() == b.sum() or abs(int(a.sum()) - int(b.sum())) <= 1e-6 * a.sum()
     diff = np.abs(a - b).sum()
     assert diff <= 1e-5 * a.sum(), (diff, a.sum())
+
+
+def test_native_lc_derivatives_match_host_differences():
+    """rtmc_lc_derivatives against batch.lc_derivatives (both central differences
+    of sigma and Sigma at lc (1 +- 1e-4)), 2-D and 3-D."""
+    from radiativetransfermontecarlo_amd import batch as rb
+    g3, s3, base3, o3, params, Ns = cases.config4()
+    mats = [(base3, 3, params[i]) for i in (0, 11, 40)]
+    mats.append((_acoustic(2, 'gaussian', lc=0.3), 2, (0.07, 0.3, math.inf)))
+    plist = [nm.params_for(b, d, *p, knot_threshold=0.0) for b, d, p in mats]
+    g, dS = nm.lc_derivatives(plist, 2049)
+    for i, (b, d, p) in enumerate(mats):
+        m = rb.prepare_material(b, d, *p, knot_threshold=0.0)
+        gp, dSp = rb.lc_derivatives(m, d)
+        assert np.max(np.abs(g[i] - gp)) <= 1e-9 * np.max(np.abs(gp))
+        assert dS[i] == pytest.approx(dSp, rel=1e-9)
+
+
+def test_native_batch_entry_points_reject_bad_items():
+    p = nm.params_from(_acoustic(3, 'exp'), 3)
+    e = nm.params_from(cases.config2()[2], 2)
+    with pytest.raises(Exception, match="material 1: lc derivatives are implemented for acoustic media"):
+        nm.lc_derivatives([p, e])
+    q = nm.params_from(_acoustic(3, 'exp'), 3)
+    q.lc = -1.0
+    with pytest.raises(Exception, match="material 0: frequency and correlation length"):
+        nm.lc_derivatives([q, p])
+
+
+def _download(ptr, n):
+    """n doubles at device address ptr (hipMemcpy device to host)."""
+    import ctypes as C
+    import torch
+    torch.cuda.synchronize()
+    hip = C.CDLL("libamdhip64.so")
+    out = np.zeros(int(n))
+    rc = hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(ptr), C.c_size_t(8 * int(n)), C.c_int(2))
+    assert rc == 0
+    return out
+
+
+@pytest.mark.gpu
+def test_native_batched_preparation_equals_single(gpu):
+    """rtmc_prepare_materials (one material per thread, one device block) gives
+    bit for bit the tables of rtmc_prepare_material (threaded angle grid)."""
+    g3, s3, base3, o3, params, Ns = cases.config4()
+    plist = [nm.params_for(base3, 3, *params[i], knot_threshold=0.0) for i in range(6)]
+    plist.append(nm.params_from(cases.config2()[2], 2))
+    many = nm.NativeMaterials(plist)
+
+    for i, p in enumerate(plist):
+        one = nm.NativeMaterial(p)
+        a, b = many.items[i].c, one.c
+        assert (a.acoustics, a.vp, a.vs, a.P2P, a.S2S, a.QP, a.QS) == (b.acoustics, b.vp, b.vs, b.P2P, b.S2S, b.QP, b.QS)
+        assert tuple(a.mft) == tuple(b.mft)
+        for c in range(1 if p.acoustics else 4):
+            ta, tb = a.invcdf[c], b.invcdf[c]
+            assert ta.n == tb.n and ta.extrap == tb.extrap
+            for pa, pb in ((ta.x, tb.x), (ta.y, tb.y)):
+                ha = _download(pa, ta.n)
+                hb = _download(pb, tb.n)
+                assert np.array_equal(ha, hb), (i, c)
+        one.close()
+    many.close()
