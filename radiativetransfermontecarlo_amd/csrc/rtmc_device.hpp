@@ -94,6 +94,9 @@ __device__ __forceinline__ double acosd(double c) { return acos(c) * 180.0 / kPi
 __device__ __forceinline__ double acosd(float c) { return (double)(acosf(c) * (180.0f / 3.14159265f)); }
 __device__ __forceinline__ void sc(double a, double* s, double* c) { sincos(a, s, c); }
 __device__ __forceinline__ void sc(float a, float* s, float* c) { sincosf(a, s, c); }
+// sin and cos of 2 pi u (the reduction by whole turns is exact)
+__device__ __forceinline__ void sc2pi(double u, double* s, double* c) { sincospi(2.0 * u, s, c); }
+__device__ __forceinline__ void sc2pi(float u, float* s, float* c) { sincospif(2.0f * u, s, c); }
 
 // ------------------------------------------------------------------
 // griddedInterpolant(x, y)(q): linear inside, linear or nearest outside
@@ -392,18 +395,24 @@ __device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, cons
 // propagateParticle.m:33-72).  bit2d: the 2-D randi([0 1]) sign bit;
 // uphi: 3-D azimuth uniform; uth: polar-angle uniform; uconv: the P2P (P)
 // or S2S (S) uniform.  Returns theta (from the double-precision table).
-template <int DIM, typename T, typename TabF>
+// FAST (the fused kernel): sin and cos of the azimuth by sincospi of 2 uphi
+// instead of sincos of 2 pi uphi (the same angle, without the reduction).
+template <int DIM, typename T, typename TabF, bool FAST = false>
 __device__ __forceinline__ double scatter_angles_t(const MatDev& M, bool acoustic, T d[3], T q[3], int& isP,
                                                    bool bit2d, double uphi, double uth, double uconv, TabF lookup) {
   if (DIM == 2) {
     T s = bit2d ? T(-1) : T(1);
     for (int k = 0; k < 3; ++k) q[k] = s * q[k];
   } else {
-    T phi = (T)(kTwoPi * uphi);
     T c[3];
     cross3(d, q, c);
     T sp, cp;
-    sc(phi, &sp, &cp);
+    if (FAST) {
+      sc2pi((T)uphi, &sp, &cp);
+    } else {
+      T phi = (T)(kTwoPi * uphi);
+      sc(phi, &sp, &cp);
+    }
     for (int k = 0; k < 3; ++k) q[k] = cp * q[k] + sp * c[k];
   }
   if (acoustic) return lookup(0, uth);
@@ -442,6 +451,21 @@ __device__ __forceinline__ void rotate_renorm(T d[3], T q[3], T th) {
   for (int k = 0; k < 3; ++k) {
     d[k] = nd[k];
     q[k] = np[k];
+  }
+}
+
+// rotate_renorm with one division per vector (the fused kernel)
+template <typename T>
+__device__ __forceinline__ void rotate_renorm_fast(T d[3], T q[3], T th) {
+  T st, ct;
+  sc(th, &st, &ct);
+  T nd[3], np[3];
+  for (int k = 0; k < 3; ++k) nd[k] = ct * d[k] + st * q[k];
+  for (int k = 0; k < 3; ++k) np[k] = -st * d[k] + ct * q[k];
+  const T id = T(1) / sqrt(sumsq3(nd)), ip = T(1) / sqrt(sumsq3(np));
+  for (int k = 0; k < 3; ++k) {
+    d[k] = nd[k] * id;
+    q[k] = np[k] * ip;
   }
 }
 

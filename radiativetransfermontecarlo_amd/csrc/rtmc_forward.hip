@@ -117,6 +117,10 @@ int make_prob(const rtmc_material& m, const rtmc_source& s, int64_t n, uint64_t 
   double omega = 2.0 * kPi * m.frequency;
   P->lamK[0] = (P->useQ && std::isfinite(QP)) ? omega * dts / QP : 0.0;
   P->lamK[1] = (P->useQ && std::isfinite(QS)) ? omega * dts / QS : 0.0;
+  for (int k = 0; k < 2; ++k) {
+    P->ilamS[k] = P->lamS[k] > 0.0 ? 1.0 / P->lamS[k] : 0.0;
+    P->ilamK[k] = P->lamK[k] > 0.0 ? 1.0 / P->lamK[k] : 0.0;
+  }
   // scheduling weight: an event costs about six deposits
   double T = dt * (double)(n_times - 1);
   double ev = std::isfinite(mftmin) && mftmin > 0.0 ? T / mftmin : 0.0;

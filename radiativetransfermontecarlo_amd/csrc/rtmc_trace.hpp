@@ -87,6 +87,8 @@ struct ProbDev {
   double dt_sub;
   double lamS[2];     // scattering: dt_s / mft(mode) (per sub-step)
   double lamK[2];     // removal: omega dt_s / Q(mode) (per sub-step)
+  double ilamS[2];    // 1 / lamS (0 where lamS is 0)
+  double ilamK[2];    // 1 / lamK (0 where lamK is 0)
   int useQ;
   double cost;        // estimated work per particle (scheduling weight)
   uint32_t* E;        // histogram (forward)
@@ -190,12 +192,13 @@ struct Rng {
 };
 
 // index of the next sub-step with a success, trials of probability 1-exp(-lam)
+// (ilam = 1 / lam)
 template <typename R>
-__device__ __forceinline__ int64_t geom_after(Rng& r, double lam) {
+__device__ __forceinline__ int64_t geom_after(Rng& r, double lam, double ilam) {
   double u = r.u<R>();
   if (!(lam > 0.0)) return kInf;
   double e = (sizeof(R) == 8) ? -log(u) : (double)(-logf((float)u));
-  double g = ceil(e / lam);
+  double g = ceil(e * ilam);
   if (g < 1.0) g = 1.0;
   if (g > 1e15) return kInf;
   return (int64_t)g;
@@ -673,8 +676,8 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               if (QUANT) {
                 h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
                 kb = 0;
-                Ks = geom_after<R>(rng, P.lamS[isP ? 0 : 1]);
-                Kk = P.useQ ? geom_after<R>(rng, P.lamK[isP ? 0 : 1]) : kInf;
+                Ks = geom_after<R>(rng, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
+                Kk = P.useQ ? geom_after<R>(rng, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]) : kInf;
                 next_event();
               } else {
                 h = (R)(isP ? M.vp : M.vs);
@@ -733,26 +736,26 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                   // mode changed at the boundary: this sub-step's scattering test and
                   // the removal tests from the next one use the new mode
                   h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
-                  Ks = j - 1 + geom_after<R>(rng, P.lamS[isP ? 0 : 1]);
-                  Kk = P.useQ ? j + geom_after<R>(rng, P.lamK[isP ? 0 : 1]) : kInf;
+                  Ks = j - 1 + geom_after<R>(rng, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
+                  Kk = P.useQ ? j + geom_after<R>(rng, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]) : kInf;
                 }
                 if (Ks == j) {
                   const int before = isP;
                   double uphi = rng.u<R>();
                   double uth = rng.u<R>();
                   double uconv = AC ? 0.0 : rng.u<R>();
-                  double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
+                  double th = scatter_angles_t<DIM, R, decltype(lookup), true>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
                   if (ADJ) {
                     nsc += 1;
                     gsum += glc_eval(P, th) - P.dlogS;
                   }
-                  rotate_renorm(d, q, (R)th);
+                  rotate_renorm_fast(d, q, (R)th);
                   nscat += 1;
                   if (!AC && isP != before) {
                     h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
-                    if (P.useQ) Kk = j + geom_after<R>(rng, P.lamK[isP ? 0 : 1]);
+                    if (P.useQ) Kk = j + geom_after<R>(rng, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]);
                   }
-                  Ks = j + geom_after<R>(rng, P.lamS[isP ? 0 : 1]);
+                  Ks = j + geom_after<R>(rng, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
                 }
                 xb[0] = x[0];
                 xb[1] = x[1];
@@ -778,12 +781,12 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             double uphi = rng.u<R>();
             double uth = rng.u<R>();
             double uconv = AC ? 0.0 : rng.u<R>();
-            double th = scatter_angles_t<DIM>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
+            double th = scatter_angles_t<DIM, R, decltype(lookup), true>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
             if (ADJ) {
               nsc += 1;
               gsum += glc_eval(P, th) - P.dlogS;
             }
-            rotate_renorm(d, q, (R)th);
+            rotate_renorm_fast(d, q, (R)th);
             dbin = dir_bin(p, bs, sedges, DIM, d[2]);
             nscat += 1;
             h = (R)(isP ? M.vp : M.vs);
