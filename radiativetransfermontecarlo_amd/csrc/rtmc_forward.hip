@@ -44,6 +44,34 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
   }
 }
 
+// Global guide of every inverse-CDF table of the launch: cell c of table t of
+// problem p gets the largest knot index k <= n-2 with x[k] <= c/G (0 when
+// c/G < x[0]), by a binary search per cell (Chen and Asau's guide table, here
+// with about one cell per knot so that a lookup brackets one interval).
+// blockIdx.y = p * ntab + t.
+__global__ __launch_bounds__(256) void k_guide_build(const ProbDev* __restrict__ probs, int ntab, int log2G) {
+  const ProbDev& P = probs[blockIdx.y / ntab];
+  const int t = blockIdx.y % ntab;
+  const TableDev& T = P.M.inv[t];
+  const int64_t G = (int64_t)1 << log2G;
+  int* g = const_cast<int*>(P.gg) + (size_t)t * (size_t)(G + 1);
+  const double* __restrict__ x = T.x;
+  const int64_t n = T.n;
+  const double invG = 1.0 / (double)G;  // exact
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= G; c += (int64_t)gridDim.x * blockDim.x) {
+    const double q = (double)c * invG;
+    int64_t lo = 0, hi = n - 1;
+    if (q >= x[0]) {
+      while (hi - lo > 1) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (x[mid] <= q) lo = mid;
+        else hi = mid;
+      }
+    }
+    g[c] = (int)lo;
+  }
+}
+
 namespace {
 
 // Launch configuration resolved from the shared part of a batch.
@@ -314,7 +342,35 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       for (int i = 0; i < np; ++i) probs[i].T = (uint32_t*)tbuf + (size_t)pl.tsize * (size_t)i;
     }
   }
-  RTMC_HIP(hipMallocAsync(&buf, pbytes + wbytes + abytes, st));
+  // global guides of the inverse-CDF tables: about one cell per knot, G a power of two
+  void* gbuf = nullptr;
+  int64_t nmax = 2;
+  for (int i = 0; i < np; ++i)
+    for (int t = 0; t < pl.p.ntab; ++t) nmax = std::max<int64_t>(nmax, probs[i].M.inv[t].n);
+  int lg = 10;
+  while (lg < 22 && ((int64_t)1 << lg) < nmax) ++lg;
+  const char* gge = getenv("RTMC_GLOBAL_GUIDE");
+  const bool want_gg = !(gge && atoi(gge) == 0) && nmax <= ((int64_t)1 << 31) - 2;
+  const size_t gcells = ((size_t)1 << lg) + 1;
+  const size_t gbytes = sizeof(int) * gcells * (size_t)pl.p.ntab * (size_t)np;
+  pl.p.gg_log2 = 0;
+  const int lds_guide = pl.p.guide_n;  // restored below: the plan may be run again
+  if (want_gg) {
+    if (keep_pool_memory() == RTMC_OK && hipMallocAsync(&gbuf, gbytes, st) == hipSuccess) {
+      for (int i = 0; i < np; ++i) probs[i].gg = (const int*)gbuf + gcells * (size_t)pl.p.ntab * (size_t)i;
+      pl.p.gg_log2 = lg;
+      pl.p.guide_n = 0;  // the LDS guide is not needed
+    } else {
+      gbuf = nullptr;
+      (void)hipGetLastError();  // fall back to the LDS guide
+    }
+  }
+  if (hipMallocAsync(&buf, pbytes + wbytes + abytes, st) != hipSuccess) {
+    if (gbuf) (void)hipFreeAsync(gbuf, st);
+    if (tbuf) (void)hipFreeAsync(tbuf, st);
+    set_error("hipMallocAsync of %zu bytes of problem data failed", pbytes + wbytes + abytes);
+    return RTMC_ERR_NOMEM;
+  }
   char* b = (char*)buf;
   ProbDev* dprobs = (ProbDev*)b;
   pl.p.work = (unsigned long long*)(b + pbytes);
@@ -326,6 +382,16 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   if (e != hipSuccess) {
     set_error("problem upload failed: %s", hipGetErrorString(e));
     rc = RTMC_ERR_HIP;
+  }
+  if (rc == RTMC_OK && gbuf) {
+    const unsigned gx = (unsigned)std::min<size_t>((gcells + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_guide_build, dim3(gx, (unsigned)(np * pl.p.ntab)), dim3(256), 0, st,
+                       (const ProbDev*)dprobs, pl.p.ntab, lg);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+      set_error("k_guide_build launch failed: %s", hipGetErrorString(le));
+      rc = RTMC_ERR_HIP;
+    }
   }
   if (rc == RTMC_OK) {
     timer_mark(st, 0);
@@ -355,10 +421,14 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       }
     }
   }
+  pl.p.guide_n = lds_guide;
+  pl.p.gg_log2 = 0;
   hipError_t fe = hipFreeAsync(buf, st);
   hipError_t fe2 = tbuf ? hipFreeAsync(tbuf, st) : hipSuccess;
-  if (rc == RTMC_OK && (fe != hipSuccess || fe2 != hipSuccess)) {
-    set_error("hipFreeAsync failed: %s", hipGetErrorString(fe != hipSuccess ? fe : fe2));
+  hipError_t fe3 = gbuf ? hipFreeAsync(gbuf, st) : hipSuccess;
+  if (rc == RTMC_OK && (fe != hipSuccess || fe2 != hipSuccess || fe3 != hipSuccess)) {
+    set_error("hipFreeAsync failed: %s",
+              hipGetErrorString(fe != hipSuccess ? fe : (fe2 != hipSuccess ? fe2 : fe3)));
     return RTMC_ERR_HIP;
   }
   return rc;

@@ -105,6 +105,10 @@ struct ProbDev {
   double kq;          // omega dt_s / Q^2 (0 when Q is infinite)
   const double* glc;  // d log sigma(theta) / d lc at theta_k = k pi / (n_glc - 1)
   int64_t n_glc;
+  // guide of each inverse-CDF table in global memory (LaunchParams::gg_log2 > 0):
+  // table t's cell c (u in [c/G, (c+1)/G)) at gg[t * (G + 1) + c] holds the largest
+  // knot index k <= n-2 with x[k] <= c/G (0 below x[0])
+  const int* gg;
 };
 
 enum { HM_GLOBAL = 0, HM_LDS = 1, HM_TMAJOR = 2 };
@@ -125,7 +129,8 @@ struct LaunchParams {
   int chunk;          // particles fetched per counter access (per wavefront)
   int batch;          // lanes waiting for an event before a wavefront takes an event step
   int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
-  int guide_n;        // guide cells per table (power of two; 0 = plain binary search)
+  int guide_n;        // LDS guide cells per table (power of two; 0 = none)
+  int gg_log2;        // log2 of the global guide's cells per table (0 = none; then the LDS guide)
   int64_t hist_size;  // words of the LDS histogram / weight table (0 = global memory)
   int nt_pad;         // HM_TMAJOR: n_times rounded up to even (time stride of T is 1)
   int hot_slots;      // HM_TMAJOR: slots of the LDS cache of hot counters (power of two, or 0)
@@ -579,6 +584,18 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     // table_eval), bracketed by the LDS guide when present
     auto lookup = [&](int t, double u) -> double {
       const TableDev& T = M.inv[t];
+      if (p.gg_log2 > 0) {
+        // cell of u in [0, 1] (u * G is exact for G a power of two); one 8-byte read
+        // brackets the knot, usually to one interval
+        const int G = 1 << p.gg_log2;
+        if (u >= 0.0 && u <= 1.0) {
+          int c = (int)(u * (double)G);
+          c = c > G - 1 ? G - 1 : c;
+          const int* g = P.gg + (size_t)t * (size_t)(G + 1) + c;
+          return table_eval_bracket(T.x, T.y, T.n, T.extrap, u, g[0], (int64_t)g[1] + 1);
+        }
+        return table_eval(T.x, T.y, T.n, T.extrap, u);
+      }
       if (p.guide_n > 0 && u >= s_tx0[t] && u <= s_tx1[t]) {
         const int G = p.guide_n;
         int c = (int)(u * (double)G);
