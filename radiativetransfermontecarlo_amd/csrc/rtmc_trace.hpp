@@ -812,7 +812,14 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
         }
       } else {
         // ---------------- DEPOSIT step ----------------
-        if (fly) {
+        // Up to DEPS observations per lane (each lane stops at its next event or
+        // its last observation): the step's scheduling is shared by them.
+        constexpr int DEPS = TMAJ ? 1 : 2;
+#pragma unroll
+        for (int rep = 0; rep < DEPS; ++rep) {
+        const bool dep_now =
+            rep == 0 ? fly : (has && (QUANT ? (kev > (int64_t)it * P.nsub) : (ts > (double)it * p.dt_obs)));
+        if (dep_now) {
           R X[3];
           if (QUANT) {
             const R m = (R)((int64_t)it * P.nsub - kb);
@@ -891,6 +898,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             has = false;
             need = true;
           }
+        }
         }
       }
     }
