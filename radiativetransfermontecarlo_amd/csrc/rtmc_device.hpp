@@ -155,6 +155,35 @@ __device__ __forceinline__ double table_eval_bracket(const double* __restrict__ 
   return y[k] + t * (y[k + 1] - y[k]);
 }
 
+// table_eval with the interval bracketed exactly, x[lo] <= q < x[hi] (a guide
+// built from this non-decreasing x): no widening checks, so the knots are read
+// once, after the few search steps (the bracket usually holds one interval).
+__device__ __forceinline__ double table_eval_exact_bracket(const double* __restrict__ x,
+                                                           const double* __restrict__ y, int64_t n, int extrap,
+                                                           double q, int64_t lo, int64_t hi) {
+  if (q != q) return q;
+  int64_t k;
+  if (q < x[0]) {
+    if (extrap == RTMC_EXTRAP_NEAREST) return y[0];
+    k = 0;
+  } else if (q > x[n - 1]) {
+    if (extrap == RTMC_EXTRAP_NEAREST) return y[n - 1];
+    k = n - 2;
+  } else {
+    lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+    hi = hi > n - 1 ? n - 1 : (hi <= lo ? lo + 1 : hi);
+    while (hi - lo > 1) {
+      int64_t mid = lo + ((hi - lo) >> 1);
+      if (x[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    k = lo;
+  }
+  const double x0 = x[k], x1 = x[k + 1], y0 = y[k], y1 = y[k + 1];
+  double t = (q - x0) / (x1 - x0);
+  return y0 + t * (y1 - y0);
+}
+
 struct TableDev {
   const double* x;
   const double* y;

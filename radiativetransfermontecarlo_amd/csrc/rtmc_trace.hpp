@@ -592,7 +592,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
           int c = (int)(u * (double)G);
           c = c > G - 1 ? G - 1 : c;
           const int* g = P.gg + (size_t)t * (size_t)(G + 1) + c;
-          return table_eval_bracket(T.x, T.y, T.n, T.extrap, u, g[0], (int64_t)g[1] + 1);
+          return table_eval_exact_bracket(T.x, T.y, T.n, T.extrap, u, g[0], (int64_t)g[1] + 1);
         }
         return table_eval(T.x, T.y, T.n, T.extrap, u);
       }
