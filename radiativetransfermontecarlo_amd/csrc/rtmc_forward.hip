@@ -48,10 +48,12 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
 // problem p gets the largest knot index k <= n-2 with x[k] <= c/G (0 when
 // c/G < x[0]), by a binary search per cell (Chen and Asau's guide table, here
 // with about one cell per knot so that a lookup brackets one interval).
-// blockIdx.y = p * ntab + t.
-__global__ __launch_bounds__(256) void k_guide_build(const ProbDev* __restrict__ probs, int ntab, int log2G) {
-  const ProbDev& P = probs[blockIdx.y / ntab];
-  const int t = blockIdx.y % ntab;
+// row0 + blockIdx.y = p * ntab + t.
+__global__ __launch_bounds__(256) void k_guide_build(const ProbDev* __restrict__ probs, int ntab, int log2G,
+                                                     int row0) {
+  const int row = row0 + (int)blockIdx.y;
+  const ProbDev& P = probs[row / ntab];
+  const int t = row % ntab;
   const TableDev& T = P.M.inv[t];
   const int64_t G = (int64_t)1 << log2G;
   int* g = const_cast<int*>(P.gg) + (size_t)t * (size_t)(G + 1);
@@ -385,12 +387,16 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   }
   if (rc == RTMC_OK && gbuf) {
     const unsigned gx = (unsigned)std::min<size_t>((gcells + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_guide_build, dim3(gx, (unsigned)(np * pl.p.ntab)), dim3(256), 0, st,
-                       (const ProbDev*)dprobs, pl.p.ntab, lg);
-    hipError_t le = hipGetLastError();
-    if (le != hipSuccess) {
-      set_error("k_guide_build launch failed: %s", hipGetErrorString(le));
-      rc = RTMC_ERR_HIP;
+    const int rows = np * pl.p.ntab;
+    for (int r0 = 0; r0 < rows && rc == RTMC_OK; r0 += 65535) {  // grid.y limit
+      const int nr = std::min(65535, rows - r0);
+      hipLaunchKernelGGL(k_guide_build, dim3(gx, (unsigned)nr), dim3(256), 0, st, (const ProbDev*)dprobs,
+                         pl.p.ntab, lg, r0);
+      hipError_t le = hipGetLastError();
+      if (le != hipSuccess) {
+        set_error("k_guide_build launch failed: %s", hipGetErrorString(le));
+        rc = RTMC_ERR_HIP;
+      }
     }
   }
   if (rc == RTMC_OK) {
