@@ -264,3 +264,23 @@ def test_radial_bins_fine_global_edges_match_coarse(gpu, frame, d):
     summed = fine.reshape(100, 50, *fine.shape[1:]).sum(axis=1)
     assert summed.sum() == coarse.sum() > 0
     assert np.abs(summed - coarse).sum() <= 2e-5 * coarse.sum()
+
+
+def test_radial_source_statistics(gpu):
+    """A user radial source law (source.radial, initializeParticle.m:22-27) in
+    the tracing kernel's birth, against the oracle's births and flights."""
+    class Law:
+        Rmax = 1.0
+
+        def __call__(self, r):
+            return 1.0 + 0.0 * r  # uniform density in r^(d-1) dr: a uniform ball
+
+    geometry = {'dimension': 3, 'frame': 'spherical'}
+    source = {'lambda': 0.2, 'radial': Law()}
+    material = cases.iso_acoustic(3)
+    observation = {'x': np.linspace(0, 2, 21), 'time': np.arange(0, 0.3 + 1e-9, 0.05)}
+    O, Eg, Eo, cnt = _run_both(geometry, source, material, observation, 500_000, 50_000)
+    assert_same_distribution(Eg, 500_000, Eo, 50_000, "radial source")
+    # at t = 0.05 (v = 2) every particle lies within 1 + 0.1 of the origin
+    first = Eg.reshape(O.shape, order='F')[:, 0, 1, 0]
+    assert first[11:].sum() == 0 and first.sum() == 500_000

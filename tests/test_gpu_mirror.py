@@ -72,8 +72,18 @@ def _replay_values(rng, steps, slots, N, d, scat_p=0.3, bit_slots=()):
     return v
 
 
+class _RadialLaw:
+    """A user radial source law f(r) on [0, Rmax] (source.radial, initializeParticle.m:22-27)."""
+    Rmax = 0.8
+
+    def __call__(self, r):
+        return np.exp(-(r / 0.3) ** 2) * (1.0 + r)
+
+
 SOURCES = [
     (3, {'lambda': 0.2, 'position': [0.1, -0.2, 0.3]}),
+    (3, {'lambda': 0.2, 'radial': _RadialLaw()}),
+    (2, {'lambda': 0.05, 'radial': _RadialLaw()}),
     (3, {'lambda': 0.2, 'direction': 'outgoing'}),
     (3, {'lambda': 0.2, 'direction': 'upper'}),
     (2, {'lambda': 0.05, 'position': [1, 2]}),
@@ -89,7 +99,8 @@ def test_init_particles(gpu, d, src, kind):
     rng = np.random.default_rng(1)
     if kind == "replay":
         vals = rng.random((1, 8, N))
-        vals[0, 0] = rng.normal(size=N)
+        if 'radial' not in src:
+            vals[0, 0] = rng.normal(size=N)  # randn radius; a user radial law takes a uniform
         hd = orc.HostDraws.recorded(vals)
         gd = solver.Draws.recorded(torch.tensor(vals, device=gpu))
     else:
