@@ -304,35 +304,42 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
 // FP64 value).
 template <int DIM>
 __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x[3], double d[3], double q[3]) {
+  // The same draws and the same angles as initializeParticle.m:17-147; each
+  // angle 2 pi u enters through sincospi(2u), and a polar angle acos(c) through
+  // its cosine c and sine sqrt(1 - c^2) (no acos, no sincos).
   if (S.type == RTMC_SRC_POINT) {
     double r;
     if (S.radial.n == 0) {
       double u1 = rng.next(), u2 = rng.next();
-      double g = sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2);
+      double g = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
       r = fabs(g * S.lambda);  // initializeParticle.m:23
     } else {
       r = table_eval(S.radial.x, S.radial.y, S.radial.n, S.radial.extrap, rng.next());  // :25-27
     }
-    double phi0, phi;
+    // polar angles as (cos, sin); azimuths as turns
+    double cp0, sp0, cp, sp;
     if (DIM == 2) {
-      phi0 = kPi / 2;
-      phi = kPi / 2;
+      cp0 = cp = cos(kPi / 2);  // 6.12e-17, as MATLAB's cos(pi/2)
+      sp0 = sp = 1.0;
     } else {
-      phi0 = acos(-1.0 + 2.0 * rng.next());
-      phi = acos(-1.0 + 2.0 * rng.next());
+      cp0 = -1.0 + 2.0 * rng.next();
+      sp0 = sqrt(fmax(0.0, 1.0 - cp0 * cp0));
+      cp = -1.0 + 2.0 * rng.next();
+      sp = sqrt(fmax(0.0, 1.0 - cp * cp));
     }
-    double theta0 = kTwoPi * rng.next();
-    double theta = kTwoPi * rng.next();
+    double t0 = rng.next();
+    double t = rng.next();
     if (S.direction == RTMC_DIR_OUTGOING) {
-      theta = theta0;
-      phi = phi0;
+      t = t0;
+      cp = cp0;
+      sp = sp0;
     } else if (S.direction == RTMC_DIR_UPPER) {
-      theta = kTwoPi * rng.next();
-      phi = acos(rng.next());
+      t = rng.next();
+      cp = rng.next();
+      sp = sqrt(fmax(0.0, 1.0 - cp * cp));
     }
-    double st, ct, sp, cp;
-    sincos(theta, &st, &ct);
-    sincos(phi, &sp, &cp);
+    double st, ct;
+    sincospi(2.0 * t, &st, &ct);
     d[0] = ct * sp;
     d[1] = st * sp;
     d[2] = cp;
@@ -340,22 +347,20 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
     q[1] = ct;
     q[2] = 0.0;
     if (DIM == 3) {
-      double alpha = kTwoPi * rng.next();
       double c[3];
       cross3(d, q, c);
       double sa, ca;
-      sincos(alpha, &sa, &ca);
+      sincospi(2.0 * rng.next(), &sa, &ca);
       for (int k = 0; k < 3; ++k) q[k] = q[k] * ca + c[k] * sa;
     }
-    double st0, ct0, sp0, cp0;
-    sincos(theta0, &st0, &ct0);
-    sincos(phi0, &sp0, &cp0);
+    double st0, ct0;
+    sincospi(2.0 * t0, &st0, &ct0);
     x[0] = S.position[0] + r * (ct0 * sp0);
     x[1] = S.position[1] + r * (st0 * sp0);
     x[2] = S.position[2] + r * cp0;
   } else {
     double u1 = rng.next(), u2 = rng.next();
-    double g = sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2);
+    double g = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
     double c_long = g * S.lambda / 2.0;
     double c1, c2;
     if (S.aperture == RTMC_APERTURE_RECTANGLE) {
@@ -363,18 +368,16 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
       c2 = (rng.next() - 0.5) * S.extent[1];
     } else {
       double rho = S.extent[0] * sqrt(rng.next());
-      double alpha = kTwoPi * rng.next();
       double sa, ca;
-      sincos(alpha, &sa, &ca);
+      sincospi(2.0 * rng.next(), &sa, &ca);
       c1 = rho * ca;
       c2 = rho * sa;
     }
     for (int k = 0; k < 3; ++k)
       x[k] = ((S.position[k] + c_long * S.normal[k]) + c1 * S.basis_u[k]) + c2 * S.basis_v[k];
     for (int k = 0; k < 3; ++k) d[k] = S.normal[k];
-    double th = rng.next() * kTwoPi;
     double s, c;
-    sincos(th, &s, &c);
+    sincospi(2.0 * rng.next(), &s, &c);
     for (int k = 0; k < 3; ++k) q[k] = c * S.basis_u[k] + s * S.basis_v[k];
   }
 }
