@@ -98,8 +98,8 @@ def cpu_baseline(args, geometry, source, material, O):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only: sharding particles across GPUs (by "
+                                   "particle_offset, histograms summed) is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
     import torch
