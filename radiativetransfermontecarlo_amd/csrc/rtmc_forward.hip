@@ -368,6 +368,8 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
     }
   }
   if (hipMallocAsync(&buf, pbytes + wbytes + abytes, st) != hipSuccess) {
+    pl.p.guide_n = lds_guide;
+    pl.p.gg_log2 = 0;
     if (gbuf) (void)hipFreeAsync(gbuf, st);
     if (tbuf) (void)hipFreeAsync(tbuf, st);
     set_error("hipMallocAsync of %zu bytes of problem data failed", pbytes + wbytes + abytes);
