@@ -500,7 +500,6 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
     const int nch = params[i].acoustics ? 1 : 4;
     for (int c = 0; c < nch; ++c) total += 2 * (size_t)h[(size_t)i].n_knots[c];
   }
-  std::vector<double> pack(total);
   void* dev = nullptr;
   if (hipMalloc(&dev, total * sizeof(double)) != hipSuccess) {
     free_all();
@@ -508,6 +507,7 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
     return RTMC_ERR_NOMEM;
   }
   size_t off = 0;
+  bool copied = true;
   for (int i = 0; i < n; ++i) {
     const rtmc_material_params* p = &params[i];
     const rtmc_material_host& hi = h[(size_t)i];
@@ -526,8 +526,8 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
     const int nch = p->acoustics ? 1 : 4;
     for (int c = 0; c < nch; ++c) {
       const size_t k = (size_t)hi.n_knots[c];
-      memcpy(pack.data() + off, hi.x[c], k * sizeof(double));
-      memcpy(pack.data() + off + k, hi.y[c], k * sizeof(double));
+      copied = copied && hipMemcpy((double*)dev + off, hi.x[c], k * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+      copied = copied && hipMemcpy((double*)dev + off + k, hi.y[c], k * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
       m.invcdf[c].n = (int64_t)k;
       m.invcdf[c].x = (const double*)dev + off;
       m.invcdf[c].y = (const double*)dev + off + k;
@@ -536,7 +536,7 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
     }
   }
   free_all();
-  if (hipMemcpy(dev, pack.data(), total * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+  if (!copied) {
     (void)hipFree(dev);
     set_error("uploading the material tables failed");
     return RTMC_ERR_NOMEM;
