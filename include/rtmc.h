@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RTMC_ABI_VERSION 2
+#define RTMC_ABI_VERSION 3
 
 /* status codes */
 #define RTMC_OK 0
@@ -168,9 +168,17 @@ typedef struct {
     uint64_t stream;            /* PHILOX: call identifier (low 32 bits used) */
 } rtmc_draws;
 
+/* rtmc_bins.direction_variable: what variable 3 measures.
+ * DIRZ: the direction's z component (observeTime.m:24-35), the reference's code;
+ * POSITION: the angle between the direction and the position vector, as
+ * README.md:209 documents it (cos of it in 3-D, the angle in 2-D; a particle
+ * at the origin has none and is not counted). */
+#define RTMC_DIRVAR_DIRZ 0
+#define RTMC_DIRVAR_POSITION 1
+
 /* Observation binning.  Variable v in {0: x / radius, 1: y / azimuth,
- * 2: z / sin(elevation), 3: direction (dir_z in 3-D, acos(dir_z) in 2-D)}
- * as computed by observeTime.m:15-45.  A particle is counted when every
+ * 2: z / sin(elevation), 3: direction (dir_z in 3-D, acos(dir_z) in 2-D, or
+ * see RTMC_DIRVAR_POSITION)} as computed by observeTime.m:15-45.  A particle is counted when every
  * variable falls inside its edges (histcounts rule: e_k <= v < e_k+1, the
  * last bin closed); bin k of variable v adds k * stride[v] to the index.
  * For the reference's two-variable histograms (initializeObservation.m:79-130)
@@ -182,6 +190,8 @@ typedef struct {
     int64_t stride[4];
     int64_t time_stride;        /* index step of one observation time */
     int64_t mode_stride;        /* index step of mode 1 (S); unused for acoustics */
+    int32_t direction_variable; /* RTMC_DIRVAR_* */
+    int32_t pad_;
 } rtmc_bins;
 
 /* Histogram placement of the fused kernel (rtmc_problem.hist_mode).

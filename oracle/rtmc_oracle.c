@@ -693,7 +693,14 @@ int orc_observe(const rtmc_geometry* G, const rtmc_bins* B, int32_t acoustics,
         if (!P->alive[i]) continue;                                           /* :138 */
         double X[3]; ld3(P->x, n, i, X);
         double vars[4];
-        orc_obs_vars(dim, G->frame, X, P->dir[2 * n + i], vars);
+        double dz = P->dir[2 * n + i];
+        if (B->direction_variable == RTMC_DIRVAR_POSITION) {
+            /* the angle to the position vector (README.md:209), as its cosine */
+            double D[3]; ld3(P->dir, n, i, D);
+            double dot = D[0] * X[0] + D[1] * X[1] + D[2] * X[2];
+            dz = dot / sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+        }
+        orc_obs_vars(dim, G->frame, X, dz, vars);
         int64_t idx = time_index * B->time_stride;
         int ok = 1;
         for (int v = 0; v < 4 && ok; ++v) {

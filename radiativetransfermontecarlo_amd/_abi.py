@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 ERR_ARG = -1
@@ -44,6 +44,10 @@ DRAWS_PHILOX = 1
 
 PREC_FP64 = 0
 PREC_FP32_FLIGHT = 1
+
+DIRVAR_DIRZ = 0
+DIRVAR_POSITION = 1
+DIRVARS = {"dir_z": DIRVAR_DIRZ, "position": DIRVAR_POSITION}
 
 CNT_SCATTER = 0
 CNT_DEPOSIT = 1
@@ -152,6 +156,8 @@ class Bins(C.Structure):
         ("stride", C.c_int64 * 4),
         ("time_stride", C.c_int64),
         ("mode_stride", C.c_int64),
+        ("direction_variable", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 

@@ -514,6 +514,16 @@ __device__ __forceinline__ void renorm(T d[3], T q[3]) {
 // them; with FAST they are sqrt(x^2 + y^2 [+ z^2]), which differs by at most
 // an ulp or two and so moves a deposit only when the radius lies that close to
 // a bin edge (positions here are far from overflow or underflow).
+// Direction variable with the direction's cosine against the position vector
+// (RTMC_DIRVAR_POSITION, README.md:209): cos = d . X / |X|, NaN at the origin.
+template <typename T>
+__device__ __forceinline__ T cos_to_position(const T d[3], const T X[3]) {
+  const T dot = d[0] * X[0] + d[1] * X[1] + d[2] * X[2];
+  return dot / sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+}
+
+// dz: the direction's z component, or (RTMC_DIRVAR_POSITION) its cosine
+// against the position vector (cos_to_position); clamped to [-1, 1] here.
 template <typename T, bool FAST = false>
 __device__ __forceinline__ void obs_vars(int dim, int frame, const T X[3], T dz, unsigned need, double v[4]) {
   T r = T(0), th = T(0), ph = T(0);

@@ -105,6 +105,8 @@ int check_shared(const rtmc_geometry& g, const rtmc_bins& B, int n_times, int ti
     if (g.dimension == 2 && v == 2) continue;
     RTMC_REQUIRE(B.n_edges[v] >= 2 && B.edges[v] != nullptr, "variable %d needs >= 2 edges", v + 1);
   }
+  RTMC_REQUIRE(B.direction_variable == RTMC_DIRVAR_DIRZ || B.direction_variable == RTMC_DIRVAR_POSITION,
+               "direction_variable must be RTMC_DIRVAR_DIRZ or RTMC_DIRVAR_POSITION");
   return RTMC_OK;
 }
 
@@ -186,6 +188,7 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   p.time_stride = B.time_stride;
   p.time_stride32 = (uint32_t)B.time_stride;
   p.mode_stride = B.mode_stride;
+  p.dirvar = B.direction_variable;
   p.n_times = n_times;
   p.dt_obs = dt;
   p.n_prob = n_prob;

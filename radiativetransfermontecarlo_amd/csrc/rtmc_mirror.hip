@@ -284,7 +284,13 @@ __global__ __launch_bounds__(kBlock) void k_observe(const rtmc_geometry G, const
   double X[3];
   ld3(P.x, n, i, X);
   double v[4];
-  obs_vars(P.dimension, G.frame, X, P.dir[2 * n + i], 15u, v);
+  double dz = P.dir[2 * n + i];
+  if (B.direction_variable == RTMC_DIRVAR_POSITION) {
+    double D[3];
+    ld3(P.dir, n, i, D);
+    dz = cos_to_position(D, X);
+  }
+  obs_vars(P.dimension, G.frame, X, dz, 15u, v);
   int64_t idx = time_index * B.time_stride;
   for (int k = 0; k < 4; ++k) {
     if (P.dimension == 2 && k == 2) continue;
@@ -445,6 +451,8 @@ extern "C" int rtmc_observe(const rtmc_geometry* geometry, const rtmc_bins* bins
   e = check_geometry(geometry);
   if (e) return e;
   RTMC_REQUIRE(bins != nullptr && E != nullptr, "bins and E must be given");
+  RTMC_REQUIRE(bins->direction_variable == RTMC_DIRVAR_DIRZ || bins->direction_variable == RTMC_DIRVAR_POSITION,
+               "direction_variable must be RTMC_DIRVAR_DIRZ or RTMC_DIRVAR_POSITION");
   for (int k = 0; k < 4; ++k) {
     if (P->dimension == 2 && k == 2) continue;
     RTMC_REQUIRE(bins->n_edges[k] >= 2 && bins->edges[k] != nullptr, "variable %d needs >= 2 edges", k + 1);

@@ -124,6 +124,7 @@ struct LaunchParams {
   uint32_t time_stride32;  // the same, for 32 x 32 -> 64-bit products
   int64_t mode_stride;
   int n_times;
+  int dirvar;         // RTMC_DIRVAR_*: the direction variable is dir_z, or the cosine to the position
   double dt_obs;
   int edges_total;    // doubles of the LDS edge copy (0: edges read from global memory)
   int chunk;          // particles fetched per counter access (per wavefront)
@@ -280,7 +281,9 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
     if (!(p.present & (1u << v))) continue;
     const double* e = sedges ? sedges + p.eoff[v] : p.edges[v];
     double lo = e[0], hi = e[p.ne[v] - 1];
-    if (p.ne[v] == 2 && trivially_inside(v, dim, p.G.frame, lo, hi)) continue;
+    // (the position-angle direction variable is checked: it is NaN at the origin)
+    if (p.ne[v] == 2 && !(v == 3 && p.dirvar == RTMC_DIRVAR_POSITION) && trivially_inside(v, dim, p.G.frame, lo, hi))
+      continue;
     bs->check |= 1u << v;
     bs->e0[v] = lo;
     bs->ehi[v] = hi;
@@ -412,11 +415,15 @@ __device__ __forceinline__ int dir_bin(const LaunchParams& p, const BinSetup& bs
 // Index of the observation's bin over the binned variables (observeTime.m:15-45
 // + histcounts), without the time and mode terms, or -1 when outside; the
 // direction's bin `dbin` comes from dir_bin.
-template <typename R>
+template <typename R, bool POS>
 __device__ __forceinline__ int bin_index(const LaunchParams& p, const BinSetup& bs,
-                                         const double* __restrict__ sedges, int dim, const R X[3], int dbin) {
+                                         const double* __restrict__ sedges, int dim, const R X[3], int dbin,
+                                         const R d[3]) {
   uint32_t idx = 0;
   if (bs.check & 8u) {
+    // the direction's bin comes with the direction (dir_bin), unless the variable
+    // is the cosine to the position vector, which moves with the particle
+    if (POS) dbin = dir_bin(p, bs, sedges, dim, cos_to_position(d, X));
     if (dbin < 0) return -1;
     idx = (uint32_t)dbin * p.stride[3];
   }
@@ -495,7 +502,8 @@ __device__ __forceinline__ int pick_problem(const LaunchParams& p, const ProbDev
 // SINK: 0 forward histogram, 1 adjoint (weighted scores)
 // LDS-histogram kernels run one 1024-lane work-group per CU (the histogram
 // takes most of the LDS); the others use 256-lane work-groups.
-template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK>
+// POS: the direction variable is the cosine to the position vector (RTMC_DIRVAR_POSITION)
+template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK, bool POS>
 __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const LaunchParams p,
                                                                      const ProbDev* __restrict__ probs) {
   constexpr bool LDSH = HM == HM_LDS;
@@ -831,7 +839,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             const R f = (R)((double)h * ((double)it * p.dt_obs - tb));
             for (int k = 0; k < 3; ++k) X[k] = xb[k] + f * d[k];
           }
-          const int sp = bin_index(p, bs, sedges, DIM, X, dbin);
+          const int sp = bin_index<R, POS>(p, bs, sedges, DIM, X, dbin, d);
           const int64_t idx = sp < 0 ? -1
                                      : (int64_t)sp + (uint64_t)(uint32_t)it * p.time_stride32 +
                                            (isP ? 0 : p.mode_stride);

@@ -64,6 +64,7 @@ class Observation:
     t: np.ndarray
     Nt: int
     widths: List[np.ndarray] = field(default_factory=list)   # dx (volume weight), dy, dz, dpsi
+    direction_variable: str = "dir_z"  # 'dir_z' (observeTime.m:24-35) or 'position' (README.md:209)
 
     @property
     def size(self) -> int:
@@ -151,9 +152,15 @@ def initializeObservation(geometry, acoustics: bool, observation, N: int, allow_
                y=y, Ny=Ny, dy=dy, binY=binY, z=z, Nz=Nz, dz=dz, binZ=binZ,
                psi=psi, Npsi=Npsi, dpsi=dpsi, binPsi=binPsi)
     edges = [binX, binY, binZ, binPsi]
+    # what `directions` measures: the reference's code bins dir_z (observeTime.m:24-35);
+    # README.md:209 describes the angle between the direction and the position vector,
+    # available as observation['direction_variable'] = 'position'
+    dv = _get(observation, 'direction_variable', 'dir_z') or 'dir_z'
+    if dv not in ('dir_z', 'position'):
+        raise ValueError("observation.direction_variable must be 'dir_z' or 'position'")
     return Observation(obs=obs, ibins=ibins, edges=edges, strides=strides, shape=shape,
                        time_stride=time_stride, mode_stride=mode_stride, n_modes=n_modes, t=t, Nt=Nt,
-                       widths=[dx, dy, dz, dpsi])
+                       widths=[dx, dy, dz, dpsi], direction_variable=dv)
 
 
 def normalize(O: Observation, E: np.ndarray, N: int):

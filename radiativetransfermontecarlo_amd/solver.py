@@ -265,6 +265,7 @@ class DeviceBins:
             b.stride[k] = int(O.strides[k])
         b.time_stride = int(O.time_stride)
         b.mode_stride = int(O.mode_stride)
+        b.direction_variable = _abi.DIRVARS[getattr(O, 'direction_variable', 'dir_z')]
         self.c = b
 
 

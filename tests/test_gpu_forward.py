@@ -284,3 +284,37 @@ def test_radial_source_statistics(gpu):
     # at t = 0.05 (v = 2) every particle lies within 1 + 0.1 of the origin
     first = Eg.reshape(O.shape, order='F')[:, 0, 1, 0]
     assert first[11:].sum() == 0 and first.sum() == 500_000
+
+
+@pytest.mark.parametrize("d", [3, 2])
+def test_position_angle_direction_variable_statistics(gpu, d):
+    """direction_variable='position' (README.md:209) in the tracing kernel: the
+    direction bin follows the particle's position at every deposit; against the
+    oracle in law."""
+    geometry = {'dimension': d, 'frame': 'spherical'}
+    source = {'lambda': 0.05, 'position': [0.0] * d}
+    material = cases.iso_acoustic(d)
+    observation = {'x': np.linspace(0, 2.5, 11), 'directions': np.linspace(0, np.pi, 7),
+                   'time': np.arange(0, 1.0 + 1e-9, 0.1), 'direction_variable': 'position'}
+    O, Eg, Eo, cnt = _run_both(geometry, source, material, observation, 500_000, 50_000)
+    assert O.direction_variable == 'position'
+    assert_same_distribution(Eg, 500_000, Eo, 50_000, f"position angle {d}-D")
+
+
+@pytest.mark.parametrize("d", [3, 2])
+def test_outgoing_ballistic_position_angle_closed_form(gpu, d):
+    """An 'outgoing' source in a medium that does not scatter within the window:
+    the direction stays along the position vector, so every deposit is in the
+    bin of cos = 1 (3-D) or angle 0 (2-D)."""
+    geometry = {'dimension': d, 'frame': 'spherical'}
+    source = {'lambda': 0.2, 'direction': 'outgoing', 'position': [0.0] * d}
+    m = MaterialClass.prepareSigma(MaterialClass(geometry, 10, True, 1, [1e-7, 1e-7], 0.0, 'exp', 0.1), d)
+    observation = {'x': np.linspace(0, 3, 31), 'directions': np.linspace(0, np.pi, 5),
+                   'time': np.arange(0, 0.5 + 1e-9, 0.05), 'direction_variable': 'position'}
+    N = 300_000
+    O = initializeObservation(geometry, True, observation, N)
+    E, cnt, _k = solver.forward_histogram(geometry, source, m, O, N, seed=8, device='cuda')
+    H = E.cpu().numpy().view(np.uint32).reshape(O.shape, order='F')
+    top = H.shape[1] - 1 if d == 3 else 0
+    assert np.all(H.sum(axis=(0, 1, 3))[1:] == N)
+    assert H[:, top, 1:, :].sum() == N * (O.Nt - 1)

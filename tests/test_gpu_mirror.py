@@ -243,6 +243,13 @@ OBS = [
     (2, 'spherical', {'x': np.linspace(0, 1.5, 13), 'y': np.linspace(-np.pi, np.pi, 9)}),
     (2, 'cartesian', {'x': np.linspace(-1, 1, 11), 'y': np.linspace(-1, 1, 11)}),
     (2, 'cylindrical', {'x': np.linspace(0, 1.5, 7), 'directions': np.linspace(0, np.pi, 5)}),
+    # the README's direction variable: the angle to the position vector
+    (3, 'spherical', {'x': np.linspace(0, 1.5, 13), 'directions': np.linspace(0, np.pi, 7),
+                      'direction_variable': 'position'}),
+    (3, 'cartesian', {'x': np.linspace(-1, 1, 9), 'directions': np.linspace(0, np.pi, 5), 'y': [-0.5, 0.5],
+                      'direction_variable': 'position'}),
+    (2, 'spherical', {'x': np.linspace(0, 1.5, 7), 'directions': np.linspace(0, np.pi, 9),
+                      'direction_variable': 'position'}),
 ]
 
 

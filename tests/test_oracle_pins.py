@@ -210,3 +210,67 @@ def test_elastic_equipartition():
     rSP = m.Sigma[1, 0]
     expect = rPS / (rPS + rSP)
     assert abs(frac_s - expect) < 5 * math.sqrt(expect * (1 - expect) / N)
+
+
+@pytest.mark.parametrize("d", [3, 2])
+def test_observe_position_angle_bruteforce(d):
+    """direction_variable='position' (README.md:209): the direction bin is the
+    cosine (3-D) or the angle (2-D) between the direction and the position
+    vector; numpy brute force, and a particle at the origin is not counted."""
+    rng = np.random.default_rng(4)
+    N = 4000
+    geo = {'dimension': d, 'frame': 'spherical'}
+    obsd = {'x': np.linspace(0, 2, 9), 'directions': np.linspace(0, np.pi, 7), 'time': [0, 1],
+            'direction_variable': 'position'}
+    O = initializeObservation(geo, True, obsd, N)
+    assert O.ibins == [1, 4] and O.direction_variable == 'position'
+    x = rng.normal(size=(3, N)) * 0.6
+    dd = rng.normal(size=(3, N))
+    if d == 2:
+        x[2] = 0.0
+        dd[2] = 0.0
+    dd /= np.linalg.norm(dd, axis=0)
+    x[:, 0] = 0.0  # at the origin: no angle
+    P = orc.HostParticles.from_arrays(d, x, dd, dd, np.ones(N, np.uint8), np.ones(N, np.uint8), np.zeros(N))
+    E = np.zeros(O.size, dtype=np.uint32)
+    orc.observe(solver.build_geometry(geo), orc.HostBins(O), True, P, E, 1)
+    r = np.sqrt(np.sum(x ** 2, axis=0))
+    with np.errstate(invalid='ignore'):
+        c = np.clip(np.sum(dd * x, axis=0) / r, -1, 1)
+    v = c if d == 3 else np.arccos(c)
+    nb = O.edges[3].size - 1
+    ref = np.zeros((8, nb), dtype=np.int64)
+    for i in range(N):
+        if not np.isfinite(v[i]):
+            continue
+        kr = min(np.searchsorted(O.edges[0], r[i], side='right') - 1, 7)
+        if not (0 <= kr < 8) or r[i] > O.edges[0][-1]:
+            continue
+        kp = min(np.searchsorted(O.edges[3], v[i], side='right') - 1, nb - 1)
+        ref[kr, kp] += 1
+    got = E.reshape(O.shape, order='F')[:, :, 1, 0]
+    assert ref.sum() == np.sum(np.isfinite(v) & (r <= 2)) and not np.isfinite(v[0])
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("d", [3, 2])
+def test_outgoing_ballistic_source_position_angle_closed_form(d):
+    """An 'outgoing' point source (direction = position vector) in a medium that
+    does not scatter within the window: every particle keeps cos = 1 (angle 0),
+    so with direction_variable='position' every deposit is in the direction bin
+    that holds it, N per observation time."""
+    geometry = {'dimension': d, 'frame': 'spherical'}
+    source = {'lambda': 0.2, 'direction': 'outgoing', 'position': [0.0] * d}
+    m = MaterialClass(geometry, 10, True, 1, [1e-7, 1e-7], 0.0, 'exp', 0.1)
+    m = MaterialClass.prepareSigma(m, d)
+    assert float(np.asarray(m.meanFreeTime).ravel()[0]) > 1e6
+    observation = {'x': np.linspace(0, 3, 31), 'directions': np.linspace(0, np.pi, 5),
+                   'time': np.arange(0, 0.5 + 1e-9, 0.05), 'direction_variable': 'position'}
+    N = 20_000
+    O = initializeObservation(geometry, True, observation, N)
+    E, nd = orc.forward(geometry, source, m, O, N, seed=3)
+    H = E.reshape(O.shape, order='F')
+    top = H.shape[1] - 1 if d == 3 else 0   # cos = 1 is the last cos-bin (3-D); angle 0 the first (2-D)
+    per_t = H.sum(axis=(0, 1, 3))
+    assert np.all(per_t[1:] == N)
+    assert H[:, top, 1:, :].sum() == N * (O.Nt - 1)
