@@ -244,6 +244,7 @@ def main():
         out["batched_fwd_adjoint_vs_plain_fd"] = {
             "workload": b["config"]["workload"], "speedup_wall": b["value"],
             "speedup_gpu_only": b["gpu_only_speedup"], "speedup_kernels_only": b["kernel_only_speedup"],
+            "speedup_wall_excluding_preparation": b["prep_excluded_speedup"],
             "batched": b["batched"], "plain_fd": b["plain_fd"],
             "gradient_agreement": b["gradient_agreement"]}
     print(json.dumps(out), flush=True)
