@@ -365,3 +365,33 @@ def test_batch_from_native_preparation_matches_host_preparation(gpu):
     gn = Bn.adjoint(w).cpu().numpy()
     gh = Bh.adjoint(w).cpu().numpy()
     np.testing.assert_allclose(gn, gh, rtol=1e-4, atol=1e-7 * np.abs(gh).max())
+
+
+@pytest.mark.parametrize("elastic", [False, True])
+def test_position_angle_histograms_do_not_depend_on_placement(gpu, elastic):
+    """direction_variable='position': the tracing kernel's separate instance gives
+    the same histogram with the LDS, direct-atomic and time-major placements and
+    both event batches (trajectories are keyed by seed and particle id), acoustic
+    and elastic."""
+    if elastic:
+        geometry = {'dimension': 2, 'frame': 'spherical'}
+        source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+        m = cases.elastic_iso(2)
+        m.Q = [40.0, 20.0]
+        m.Frequency = 1.0
+        mats = [MaterialClass.prepareSigma(m, 2)]
+        obs = {'x': np.linspace(0, 3, 13), 'directions': np.linspace(0, np.pi, 7),
+               'time': np.arange(0, 0.8 + 1e-9, 0.05), 'direction_variable': 'position'}
+        O = initializeObservation(geometry, False, obs, 1)
+    else:
+        geometry, source, base, observation = cases.config1()
+        mats = [rb.prepare_material(base, 3, c=0.08, lc=0.2, Q=300.0)]
+        obs = dict(observation, time=np.arange(0, 0.5 + 1e-9, 0.02), directions=np.linspace(0, np.pi, 9),
+                   direction_variable='position')  # 120 x 8 x 26 counters: fits in LDS
+        O = initializeObservation(geometry, True, obs, 1)
+    B = rb.Batch(geometry, O, mats, source, [150_000], [13], device='cuda')
+    E0 = B.forward()[0].clone()
+    assert int(E0.sum()) > 0
+    for cfg in ({'hist_mode': 1}, {'hist_mode': 3, 'event_batch': 8}, {'hist_mode': 2, 'event_batch': 40}):
+        B.configure(**cfg)
+        assert torch.equal(B.forward()[0], E0), cfg
