@@ -19,12 +19,22 @@ template <int DIM, typename R>
 int launch_trace(const LaunchParams& p, const ProbDev* probs, int hm, bool quant, bool ac, bool bnd, int sink,
                  size_t smem, unsigned grid, unsigned block, hipStream_t st);
 
-// E[sp + it ts + m ms] += T[(sp + m ts) nt_pad + it]: the time-major scratch
-// added into the reference layout through 64 x 64 LDS tiles (coalesced reads
-// along time, coalesced writes along the bins; zero counts are skipped).
+// E[base + sp + it ts + m ms] += T[(sp + m tsp) nt_pad + it] for sp < tsp: the
+// time-major scratch added into the reference layout through 64 x 64 LDS tiles
+// (coalesced reads along time, coalesced writes along the bins; zero counts
+// are skipped).  base: the first cell the scratch covers (a single direction
+// bin, found as the tracing kernel finds it: the bin of acos(cos(pi/2)) among
+// dir_edges; no dir_edges = 0).
 __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__ T, uint32_t* __restrict__ E,
-                                                    int64_t ts, int64_t ms, int nt, int nt_pad) {
+                                                    int64_t tsp, const double* __restrict__ dir_edges, int n_dir,
+                                                    int64_t dir_stride, int64_t ts, int64_t ms, int nt, int nt_pad) {
   __shared__ uint32_t tile[64][65];
+  int64_t base = 0;
+  if (dir_edges) {
+    const int64_t k = bin_search(dir_edges, n_dir, acos(fmin(1.0, fmax(-1.0, cos(kPi / 2)))));
+    if (k < 0) return;  // the tracing kernel deposited nothing in the scratch
+    base = k * dir_stride;
+  }
   const int64_t sp0 = (int64_t)blockIdx.x * 64;
   const int it0 = blockIdx.y * 64;
   const int64_t m = blockIdx.z;
@@ -32,7 +42,7 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
     const int r = k >> 6, c = k & 63;
     const int64_t sp = sp0 + r;
     const int it = it0 + c;
-    tile[r][c] = (sp < ts && it < nt) ? T[(sp + m * ts) * nt_pad + it] : 0u;
+    tile[r][c] = (sp < tsp && it < nt) ? T[(sp + m * tsp) * nt_pad + it] : 0u;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
@@ -40,7 +50,7 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
     const int64_t sp = sp0 + r;
     const int it = it0 + c;
     const uint32_t v = tile[r][c];
-    if (v && sp < ts && it < nt) E[sp + (int64_t)it * ts + m * ms] += v;
+    if (v && sp < tsp && it < nt) E[base + sp + (int64_t)it * ts + m * ms] += v;
   }
 }
 
@@ -75,6 +85,11 @@ __global__ __launch_bounds__(256) void k_guide_build(const ProbDev* __restrict__
 }
 
 namespace {
+
+int getenv_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
 
 // Launch configuration resolved from the shared part of a batch.
 struct Plan {
@@ -223,7 +238,12 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // time-major scratch: explicit (3), or by default whenever the histogram is in global memory
   bool tmaj = !adjoint && !ldsh && n_times >= 2 && (hist_mode == 3 || hist_mode == 0);
   p.nt_pad = n_times + (n_times & 1);
-  pl.tsize = tmaj ? B.time_stride * pl.n_modes * (int64_t)p.nt_pad : 0;
+  // 2-D, dir_z binned as the outermost variable: the scratch keeps one direction bin
+  const int64_t ndir = (int64_t)B.n_edges[3] - 1;
+  p.tcomp = tmaj && dim == 2 && B.direction_variable == RTMC_DIRVAR_DIRZ && ndir > 1 && B.stride[3] > 0 &&
+            B.stride[3] * ndir == B.time_stride && getenv_int("RTMC_TMAJOR_ONE_DIRECTION", 1) != 0;
+  p.tsp = (uint32_t)(p.tcomp ? B.stride[3] : B.time_stride);
+  pl.tsize = tmaj ? (int64_t)p.tsp * pl.n_modes * (int64_t)p.nt_pad : 0;
   pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
   pl.tmaj_auto = tmaj && hist_mode == 0;
   if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
@@ -343,6 +363,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       (void)hipGetLastError();
       pl.hm = HM_GLOBAL;  // direct global atomics need no scratch
       pl.p.hot_slots = 0;
+      pl.p.tcomp = 0;
     } else {
       for (int i = 0; i < np; ++i) probs[i].T = (uint32_t*)tbuf + (size_t)pl.tsize * (size_t)i;
     }
@@ -421,10 +442,11 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   if (rc == RTMC_OK) timer_mark(st, 1);
   if (rc == RTMC_OK && tbuf) {
     const int64_t ts = pl.p.time_stride;
-    dim3 grid((unsigned)((ts + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
+    dim3 grid((unsigned)(((int64_t)pl.p.tsp + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
     for (int i = 0; i < np && rc == RTMC_OK; ++i) {
-      hipLaunchKernelGGL(k_tmajor_add, grid, dim3(256), 0, st, (const uint32_t*)probs[i].T, probs[i].E, ts,
-                         pl.p.mode_stride, pl.p.n_times, pl.p.nt_pad);
+      hipLaunchKernelGGL(k_tmajor_add, grid, dim3(256), 0, st, (const uint32_t*)probs[i].T, probs[i].E,
+                         (int64_t)pl.p.tsp, pl.p.tcomp ? pl.p.edges[3] : (const double*)nullptr, pl.p.ne[3],
+                         (int64_t)pl.p.stride[3], ts, pl.p.mode_stride, pl.p.n_times, pl.p.nt_pad);
       hipError_t le = hipGetLastError();
       if (le != hipSuccess) {
         set_error("k_tmajor_add launch failed: %s", hipGetErrorString(le));

@@ -138,6 +138,10 @@ struct LaunchParams {
   int hot_shift;      // 64 - log2(hot_slots)
   int hot_off;        // byte offset of the cache in the dynamic LDS block
   int hot_times;      // only deposits at observation indices below this go through the cache
+  // HM_TMAJOR, 2-D with dir_z binned outermost: every particle's direction variable is
+  // acos(~0) = pi/2, so the scratch holds only that direction bin (BinSetup::tdir)
+  int tcomp;
+  uint32_t tsp;       // spatial cells per mode in the scratch (time_stride, or stride[3] when tcomp)
   int n_prob;
   unsigned long long* work;  // per-problem particle counters (zeroed)
   unsigned int* active;      // per-problem count of work-groups on it (zeroed)
@@ -257,6 +261,7 @@ struct BinSetup {
   unsigned radial;
   float e0f, invwf;
   double s_lo, s_hi;  // squared first and last edges
+  int tdir;           // LaunchParams::tcomp: the direction bin of acos(cos(pi/2)) (the scratch's only one)
 };
 
 // Is a one-bin variable's range check always passed?  (then it is skipped)
@@ -300,6 +305,11 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
   bs->invwf = (float)bs->invw[0];
   bs->s_lo = bs->e0[0] * bs->e0[0];
   bs->s_hi = bs->ehi[0] * bs->ehi[0];
+  bs->tdir = -1;
+  if (p.tcomp) {
+    const double* e3 = sedges ? sedges + p.eoff[3] : p.edges[3];
+    bs->tdir = (int)bin_search(e3, p.ne[3], acos(fmin(1.0, fmax(-1.0, cos(kPi / 2)))));
+  }
 }
 
 // Birth (initializeParticle.m:17-147), computed in FP64 for both flight
@@ -844,11 +854,20 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                                      : (int64_t)sp + (uint64_t)(uint32_t)it * p.time_stride32 +
                                            (isP ? 0 : p.mode_stride);
           if (TMAJ && !ADJ) {
-            // time-major scratch: pair an even-time deposit with the next one
-            const int64_t j = sp < 0 ? -1
-                                     : (int64_t)((uint64_t)((uint32_t)sp + (isP ? 0u : p.time_stride32)) *
-                                                     (uint32_t)p.nt_pad +
-                                                 (uint32_t)it);
+            // time-major scratch: pair an even-time deposit with the next one.  With a
+            // single-direction scratch (tcomp) a deposit in another direction bin, which
+            // the 2-D process does not produce, goes straight into E.
+            const bool tfit = !p.tcomp || dbin == bs.tdir;
+            const uint32_t tbase = p.tcomp ? (uint32_t)bs.tdir * p.stride[3] : 0u;
+            if (sp >= 0 && !tfit) {
+              __hip_atomic_fetch_add(P.E + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              ndep += 1;
+            }
+            const int64_t j = (sp < 0 || !tfit)
+                                  ? -1
+                                  : (int64_t)((uint64_t)((uint32_t)sp - tbase + (isP ? 0u : p.tsp)) *
+                                                  (uint32_t)p.nt_pad +
+                                              (uint32_t)it);
             bool cached = false;
             if (j >= 0 && p.hot_slots > 0 && it < p.hot_times) {
               const int s = (int)(((unsigned long long)j * 0x9E3779B97F4A7C15ull) >> p.hot_shift);

@@ -395,3 +395,29 @@ def test_position_angle_histograms_do_not_depend_on_placement(gpu, elastic):
     for cfg in ({'hist_mode': 1}, {'hist_mode': 3, 'event_batch': 8}, {'hist_mode': 2, 'event_batch': 40}):
         B.configure(**cfg)
         assert torch.equal(B.forward()[0], E0), cfg
+
+
+@pytest.mark.parametrize("direction,precision", [("upper", "fp64"), ("uniform", "fp32")])
+def test_one_direction_scratch_matches_direct_atomics(gpu, direction, precision):
+    """2-D with dir_z binned: the time-major scratch keeps only the direction bin
+    of pi/2.  An 'upper' source gives 2-D particles other dir_z values (their
+    deposits go straight into E), and FP32 flight computes the direction
+    variable in FP32: both equal direct global atomics."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0], 'direction': direction}
+    e = np.linspace(-3, 3, 25)
+    observation = {'x': e, 'y': e, 'directions': np.linspace(0, np.pi, 9), 'time': np.arange(0, 0.8 + 1e-9, 0.05)}
+    m = cases.elastic_iso(2)
+    m.Q = [40.0, 20.0]
+    m.Frequency = 1.0
+    m = MaterialClass.prepareSigma(m, 2)
+    O = initializeObservation(geometry, False, observation, 1, allow_nd=True)
+    out = {}
+    for hm in (1, 3):
+        E, cnt, _k = solver.forward_histogram(geometry, source, m, O, 200_003, seed=29, device='cuda', hist_mode=hm,
+                                              precision=precision)
+        out[hm] = (E.clone(), cnt.clone())
+    H = out[1][0].cpu().numpy().view(np.uint32).reshape(O.shape, order='F')
+    occupied = np.nonzero(H.sum(axis=(0, 1, 3, 4)))[0]
+    assert (len(occupied) > 1) == (direction == "upper"), occupied
+    assert torch.equal(out[3][0], out[1][0]) and torch.equal(out[3][1], out[1][1])
