@@ -103,6 +103,7 @@ struct Plan {
   int n_modes;
   int64_t tsize;  // HM_TMAJOR: words of one problem's time-major scratch
   bool tmaj_auto; // HM_TMAJOR chosen by default (falls back to HM_GLOBAL without memory)
+  bool global_guide;  // build the global guide tables (else the LDS guide, if any)
 };
 
 int check_shared(const rtmc_geometry& g, const rtmc_bins& B, int n_times, int time_steps, int precision) {
@@ -251,17 +252,23 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
               hist_bytes, max_smem);
     return RTMC_ERR_ARG;
   }
-  // the guide takes what the histogram leaves
+  // The global guide (run_plan) replaces the LDS guide, which then takes no LDS; if
+  // the global guide cannot be allocated the lookups search the whole table.
+  // RTMC_GLOBAL_GUIDE=0 keeps the LDS guide, which takes what the histogram leaves.
+  pl.global_guide = getenv_int("RTMC_GLOBAL_GUIDE", 1) != 0;
+  if (pl.global_guide) guide = 0;
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
-  // time-major mode: a cache of hot counters, 12 B a slot; the guide shrinks so that the
-  // work-group stays within 40 KB of LDS (four 256-lane work-groups per CU)
+  // time-major mode: a cache of hot counters, 12 B a slot.  2-D particles share a
+  // plane, so counters are hit far more often than in 3-D: 4096 slots there
+  // (three 256-lane work-groups per CU), 1024 in 3-D (four); the LDS guide, if
+  // any, shrinks to keep the work-group within 40 KB.
   int hot = 0;
   if (tmaj) {
-    hot = 1024;
+    hot = dim == 2 ? 4096 : 1024;
     if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
-    RTMC_REQUIRE(hot >= 0 && hot <= (1 << 14) && (hot & (hot - 1)) == 0,
-                 "RTMC_HOT_SLOTS must be 0 or a power of two <= 16384");
+    RTMC_REQUIRE(hot >= 0 && hot <= (1 << 13) && (hot & (hot - 1)) == 0,
+                 "RTMC_HOT_SLOTS must be 0 or a power of two <= 8192");
     const size_t budget = 40 * 1024;
     while (guide > 256 && edge_bytes + guide_bytes(guide) + 8 + 12 * (size_t)hot > budget) guide >>= 1;
   }
@@ -275,13 +282,16 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   for (int v = hot; v > 1; v >>= 1) p.hot_shift -= 1;
   p.hot_off = (int)((pl.smem + 7) & ~(size_t)7);
   if (hot) pl.smem = (size_t)p.hot_off + 12 * (size_t)hot;
+  RTMC_REQUIRE(pl.smem <= (size_t)max_smem, "%zu bytes of LDS per work-group exceed the %d available", pl.smem,
+               max_smem);
 
   unsigned block = block_size > 0 ? (unsigned)block_size : (ldsh ? 1024u : 256u);
   RTMC_REQUIRE(block % 64 == 0 && block <= (ldsh ? 1024u : 256u),
                "block_size must be a multiple of 64 and <= 256 (<= 1024 with the LDS histogram)");
   int64_t per_cu;
   if (ldsh) per_cu = std::max<int64_t>(1, (int64_t)max_smem / (int64_t)std::max<size_t>(pl.smem, 1));
-  else per_cu = 2048 / block;
+  else per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / block, pl.smem ? (int64_t)max_smem / (int64_t)pl.smem
+                                                                              : 2048 / block));
   RTMC_REQUIRE(ppt >= 0, "particles_per_thread must be >= 0");
   int64_t want = ppt > 0 ? (n_total + (int64_t)block * ppt - 1) / ((int64_t)block * ppt)
                          : (n_total + block - 1) / block;
@@ -375,8 +385,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
     for (int t = 0; t < pl.p.ntab; ++t) nmax = std::max<int64_t>(nmax, probs[i].M.inv[t].n);
   int lg = 10;
   while (lg < 22 && ((int64_t)1 << lg) < nmax) ++lg;
-  const char* gge = getenv("RTMC_GLOBAL_GUIDE");
-  const bool want_gg = !(gge && atoi(gge) == 0) && nmax <= ((int64_t)1 << 31) - 2;
+  const bool want_gg = pl.global_guide && nmax <= ((int64_t)1 << 31) - 2;
   const size_t gcells = ((size_t)1 << lg) + 1;
   const size_t gbytes = sizeof(int) * gcells * (size_t)pl.p.ntab * (size_t)np;
   pl.p.gg_log2 = 0;
