@@ -259,18 +259,17 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   if (pl.global_guide) guide = 0;
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
-  // time-major mode: a cache of hot counters, 12 B a slot.  2-D particles share a
-  // plane, so counters are hit far more often than in 3-D: 4096 slots there
-  // (three 256-lane work-groups per CU), 1024 in 3-D (four); the LDS guide, if
-  // any, shrinks to keep the work-group within 40 KB.
+  // time-major mode: a cache of hot counters, 8 B a slot.  2-D particles share a
+  // plane, so counters are hit far more often than in 3-D: 4096 slots there,
+  // 2048 in 3-D; the LDS guide, if any, shrinks to keep the work-group within 40 KB.
   int hot = 0;
   if (tmaj) {
-    hot = dim == 2 ? 4096 : 1024;
+    hot = dim == 2 ? 4096 : 2048;
     if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
     RTMC_REQUIRE(hot >= 0 && hot <= (1 << 13) && (hot & (hot - 1)) == 0,
                  "RTMC_HOT_SLOTS must be 0 or a power of two <= 8192");
     const size_t budget = 40 * 1024;
-    while (guide > 256 && edge_bytes + guide_bytes(guide) + 8 + 12 * (size_t)hot > budget) guide >>= 1;
+    while (guide > 256 && edge_bytes + guide_bytes(guide) + 8 + 8 * (size_t)hot > budget) guide >>= 1;
   }
   p.guide_n = guide;
   p.hist_size = ldsh ? hsize : 0;
@@ -281,7 +280,7 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   p.hot_shift = 64;
   for (int v = hot; v > 1; v >>= 1) p.hot_shift -= 1;
   p.hot_off = (int)((pl.smem + 7) & ~(size_t)7);
-  if (hot) pl.smem = (size_t)p.hot_off + 12 * (size_t)hot;
+  if (hot) pl.smem = (size_t)p.hot_off + 8 * (size_t)hot;
   RTMC_REQUIRE(pl.smem <= (size_t)max_smem, "%zu bytes of LDS per work-group exceed the %d available", pl.smem,
                max_smem);
 

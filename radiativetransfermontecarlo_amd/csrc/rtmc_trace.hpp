@@ -523,13 +523,16 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
   __shared__ BinSetup s_bs;
   __shared__ double s_tx0[4], s_tx1[4];
   __shared__ int s_cur;
+  __shared__ unsigned int s_taken;  // particles this work-group took from the current problem
   int* sguide = (int*)(smem + p.edges_total);
   uint32_t* shist = (uint32_t*)(sguide + (p.guide_n > 0 ? p.ntab * (p.guide_n + 1) : 0));
   float* sw = (float*)shist;  // adjoint: the weight table takes the histogram's place
   constexpr bool ADJ = SINK == 1;
+  // HM_TMAJOR cache of hot counters: one 64-bit LDS word per slot holding the T index
+  // (bits 24-63) and its count (bits 0-23; a counter gets one deposit per particle
+  // and the host keeps a work-group's particles below 2^24)
   constexpr unsigned long long kEmpty = ~0ull;
-  unsigned long long* hkey = (unsigned long long*)((char*)smem + p.hot_off);  // HM_TMAJOR cache keys
-  uint32_t* hcnt = (uint32_t*)(hkey + p.hot_slots);                             // and counts
+  unsigned long long* hslot = (unsigned long long*)((char*)smem + p.hot_off);
 
   if (p.edges_total > 0) {
     // gather the edge arrays into one LDS block
@@ -554,7 +557,10 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   while (true) {
-    if (threadIdx.x == 0) s_cur = pick_problem(p, probs);
+    if (threadIdx.x == 0) {
+      s_cur = pick_problem(p, probs);
+      s_taken = 0u;
+    }
     __syncthreads();
     const int cur = __builtin_amdgcn_readfirstlane(s_cur);
     if (cur < 0) break;
@@ -594,10 +600,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
       }
     }
     if (TMAJ && !ADJ) {
-      for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) {
-        hkey[k] = kEmpty;
-        hcnt[k] = 0u;
-      }
+      for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) hslot[k] = kEmpty;
     }
     __syncthreads();
 
@@ -687,8 +690,17 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
           int64_t nb0 = 0, nb1 = 0;
           if (L < n && !exhausted) {
             unsigned long long base = 0;
-            if (lane == __ffsll((long long)nm) - 1) base = atomicAdd(work, (unsigned long long)p.chunk);
-            base = __shfl(base, __ffsll((long long)nm) - 1, 64);
+            const int leader = __ffsll((long long)nm) - 1;
+            if (lane == leader) {
+              // a hot-counter slot counts at most one deposit per particle in 24 bits:
+              // past kMaxTaken particles the work-group flushes its cache and picks again
+              constexpr unsigned kMaxTaken = (1u << 24) - 65536u;
+              if (TMAJ && !ADJ && p.hot_slots > 0 && atomicAdd(&s_taken, (unsigned)p.chunk) >= kMaxTaken)
+                base = (unsigned long long)P.n;
+              else
+                base = atomicAdd(work, (unsigned long long)p.chunk);
+            }
+            base = __shfl(base, leader, 64);
             if ((int64_t)base >= P.n) {
               exhausted = true;
             } else {
@@ -871,14 +883,20 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             bool cached = false;
             if (j >= 0 && p.hot_slots > 0 && it < p.hot_times) {
               const int s = (int)(((unsigned long long)j * 0x9E3779B97F4A7C15ull) >> p.hot_shift);
-              const unsigned long long key = hkey[s];
-              if (key == (unsigned long long)j) {
+              const unsigned long long mine = (unsigned long long)j << 24;
+              const unsigned long long cur = hslot[s];
+              if (cur != kEmpty && (cur >> 24) == (unsigned long long)j) {
+                atomicAdd(hslot + s, 1ull);
                 cached = true;
-              } else if (key == kEmpty) {
-                const unsigned long long old = atomicCAS(hkey + s, kEmpty, (unsigned long long)j);
-                cached = old == kEmpty || old == (unsigned long long)j;
+              } else if (cur == kEmpty) {
+                const unsigned long long old = atomicCAS(hslot + s, kEmpty, mine | 1ull);
+                if (old == kEmpty) {
+                  cached = true;  // claimed with this deposit counted
+                } else if ((old >> 24) == (unsigned long long)j) {
+                  atomicAdd(hslot + s, 1ull);
+                  cached = true;
+                }
               }
-              if (cached) atomicAdd(hcnt + s, 1u);
             }
             if (cached) {
               if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -937,9 +955,9 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     __syncthreads();
     if (TMAJ && !ADJ) {
       for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) {
-        const unsigned long long key = hkey[k];
-        const uint32_t c = hcnt[k];
-        if (key != kEmpty && c) __hip_atomic_fetch_add(P.T + key, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long v = hslot[k];
+        const uint32_t c = (uint32_t)(v & 0xFFFFFFull);
+        if (v != kEmpty && c) __hip_atomic_fetch_add(P.T + (v >> 24), c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     if (LDSH && !ADJ) {
