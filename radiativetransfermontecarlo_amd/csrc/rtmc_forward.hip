@@ -277,6 +277,12 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   p.hot_slots = hot;
   p.hot_times = 1 << 30;
   if (const char* h = getenv("RTMC_HOT_TIMES")) p.hot_times = atoi(h);
+  {
+    const int64_t cap = (1 << 24) - 65536;  // a slot's count has 24 bits
+    const int64_t m = getenv_int("RTMC_HOT_MAX_PARTICLES", (int)cap);
+    RTMC_REQUIRE(m >= 256 && m <= cap, "RTMC_HOT_MAX_PARTICLES must be in [256, %lld]", (long long)cap);
+    p.hot_max_taken = (unsigned)m;
+  }
   p.hot_shift = 64;
   for (int v = hot; v > 1; v >>= 1) p.hot_shift -= 1;
   p.hot_off = (int)((pl.smem + 7) & ~(size_t)7);

@@ -138,6 +138,7 @@ struct LaunchParams {
   int hot_shift;      // 64 - log2(hot_slots)
   int hot_off;        // byte offset of the cache in the dynamic LDS block
   int hot_times;      // only deposits at observation indices below this go through the cache
+  unsigned hot_max_taken;  // particles a work-group takes from a problem before it flushes the cache (< 2^24)
   // HM_TMAJOR, 2-D with dir_z binned outermost: every particle's direction variable is
   // acos(~0) = pi/2, so the scratch holds only that direction bin (BinSetup::tdir)
   int tcomp;
@@ -694,8 +695,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             if (lane == leader) {
               // a hot-counter slot counts at most one deposit per particle in 24 bits:
               // past kMaxTaken particles the work-group flushes its cache and picks again
-              constexpr unsigned kMaxTaken = (1u << 24) - 65536u;
-              if (TMAJ && !ADJ && p.hot_slots > 0 && atomicAdd(&s_taken, (unsigned)p.chunk) >= kMaxTaken)
+              if (TMAJ && !ADJ && p.hot_slots > 0 && atomicAdd(&s_taken, (unsigned)p.chunk) >= p.hot_max_taken)
                 base = (unsigned long long)P.n;
               else
                 base = atomicAdd(work, (unsigned long long)p.chunk);

@@ -421,3 +421,20 @@ def test_one_direction_scratch_matches_direct_atomics(gpu, direction, precision)
     occupied = np.nonzero(H.sum(axis=(0, 1, 3, 4)))[0]
     assert (len(occupied) > 1) == (direction == "upper"), occupied
     assert torch.equal(out[3][0], out[1][0]) and torch.equal(out[3][1], out[1][1])
+
+
+def test_hot_cache_work_group_repick_keeps_counts(gpu, monkeypatch):
+    """A work-group flushes its hot-counter cache and picks the problem again after
+    RTMC_HOT_MAX_PARTICLES particles (the slots count in 24 bits); with a tiny
+    limit the time-major histogram still equals direct atomics."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-2, 2, 17)
+    observation = {'x': e, 'y': e, 'directions': np.linspace(0, np.pi, 5), 'time': np.arange(0, 0.3 + 1e-9, 0.05)}
+    m = MaterialClass.prepareSigma(cases.elastic_iso(2), 2)
+    O = initializeObservation(geometry, False, observation, 1, allow_nd=True)
+    ref, rc, _k = solver.forward_histogram(geometry, source, m, O, 400_000, seed=31, device='cuda', hist_mode=1)
+    monkeypatch.setenv("RTMC_HOT_MAX_PARTICLES", "512")
+    E, c, _k = solver.forward_histogram(geometry, source, m, O, 400_000, seed=31, device='cuda', hist_mode=3)
+    assert int(ref.sum()) > 0
+    assert torch.equal(E, ref) and torch.equal(c, rc)
