@@ -19,7 +19,8 @@ template <int DIM, typename R>
 int launch_trace(const LaunchParams& p, const ProbDev* probs, int hm, bool quant, bool ac, bool bnd, int sink,
                  size_t smem, unsigned grid, unsigned block, hipStream_t st);
 
-// E[base + sp + it ts + m ms] += T[(sp + m tsp) nt_pad + it] for sp < tsp: the
+// E[base + sp + it ts + m ms] += T[(sp + m tsp) nt_pad + it - shift] for sp < tsp,
+// it >= shift (shift 1: the scratch of pairs from odd times): the
 // time-major scratch added into the reference layout through 64 x 64 LDS tiles
 // (coalesced reads along time, coalesced writes along the bins; zero counts
 // are skipped).  base: the first cell the scratch covers (a single direction
@@ -27,7 +28,8 @@ int launch_trace(const LaunchParams& p, const ProbDev* probs, int hm, bool quant
 // dir_edges; no dir_edges = 0).
 __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__ T, uint32_t* __restrict__ E,
                                                     int64_t tsp, const double* __restrict__ dir_edges, int n_dir,
-                                                    int64_t dir_stride, int64_t ts, int64_t ms, int nt, int nt_pad) {
+                                                    int64_t dir_stride, int64_t ts, int64_t ms, int nt, int nt_pad,
+                                                    int shift) {
   __shared__ uint32_t tile[64][65];
   int64_t base = 0;
   if (dir_edges) {
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
     const int r = k >> 6, c = k & 63;
     const int64_t sp = sp0 + r;
     const int it = it0 + c;
-    tile[r][c] = (sp < tsp && it < nt) ? T[(sp + m * tsp) * nt_pad + it] : 0u;
+    tile[r][c] = (sp < tsp && it < nt && it >= shift) ? T[(sp + m * tsp) * nt_pad + it - shift] : 0u;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
@@ -245,6 +247,9 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
             B.stride[3] * ndir == B.time_stride && getenv_int("RTMC_TMAJOR_ONE_DIRECTION", 1) != 0;
   p.tsp = (uint32_t)(p.tcomp ? B.stride[3] : B.time_stride);
   pl.tsize = tmaj ? (int64_t)p.tsp * pl.n_modes * (int64_t)p.nt_pad : 0;
+  // a second scratch shifted by one time takes the pairs that start at odd times
+  // when the scratch is small (its clearing and transpose-add are then cheap)
+  p.odd_pairs = tmaj && pl.tsize * 4 <= ((int64_t)2 << 30) && getenv_int("RTMC_ODD_PAIRS", 1) != 0;
   pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
   pl.tmaj_auto = tmaj && hist_mode == 0;
   if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
@@ -364,7 +369,8 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   const size_t abytes = sizeof(unsigned int) * (size_t)np;
   void* buf = nullptr;
   void* tbuf = nullptr;
-  const size_t tbytes = sizeof(uint32_t) * (size_t)pl.tsize * (size_t)np;
+  const size_t tper = (size_t)pl.tsize * (pl.p.odd_pairs ? 2 : 1);  // words per problem: T (and T2)
+  const size_t tbytes = sizeof(uint32_t) * tper * (size_t)np;
   if (pl.hm == HM_TMAJOR) {
     int e = keep_pool_memory();
     if (e) return e;
@@ -379,8 +385,12 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       pl.hm = HM_GLOBAL;  // direct global atomics need no scratch
       pl.p.hot_slots = 0;
       pl.p.tcomp = 0;
+      pl.p.odd_pairs = 0;
     } else {
-      for (int i = 0; i < np; ++i) probs[i].T = (uint32_t*)tbuf + (size_t)pl.tsize * (size_t)i;
+      for (int i = 0; i < np; ++i) {
+        probs[i].T = (uint32_t*)tbuf + tper * (size_t)i;
+        probs[i].T2 = pl.p.odd_pairs ? probs[i].T + pl.tsize : nullptr;
+      }
     }
   }
   // global guides of the inverse-CDF tables: about one cell per knot, G a power of two
@@ -458,9 +468,12 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
     const int64_t ts = pl.p.time_stride;
     dim3 grid((unsigned)(((int64_t)pl.p.tsp + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
     for (int i = 0; i < np && rc == RTMC_OK; ++i) {
-      hipLaunchKernelGGL(k_tmajor_add, grid, dim3(256), 0, st, (const uint32_t*)probs[i].T, probs[i].E,
-                         (int64_t)pl.p.tsp, pl.p.tcomp ? pl.p.edges[3] : (const double*)nullptr, pl.p.ne[3],
-                         (int64_t)pl.p.stride[3], ts, pl.p.mode_stride, pl.p.n_times, pl.p.nt_pad);
+      for (int sh = 0; sh <= (pl.p.odd_pairs ? 1 : 0); ++sh) {
+        hipLaunchKernelGGL(k_tmajor_add, grid, dim3(256), 0, st, (const uint32_t*)(sh ? probs[i].T2 : probs[i].T),
+                           probs[i].E, (int64_t)pl.p.tsp, pl.p.tcomp ? pl.p.edges[3] : (const double*)nullptr,
+                           pl.p.ne[3], (int64_t)pl.p.stride[3], ts, pl.p.mode_stride, pl.p.n_times, pl.p.nt_pad,
+                           sh);
+      }
       hipError_t le = hipGetLastError();
       if (le != hipSuccess) {
         set_error("k_tmajor_add launch failed: %s", hipGetErrorString(le));

@@ -93,6 +93,7 @@ struct ProbDev {
   double cost;        // estimated work per particle (scheduling weight)
   uint32_t* E;        // histogram (forward)
   uint32_t* T;        // HM_TMAJOR scratch: T[(spatial + mode * time_stride) * nt_pad + it]
+  uint32_t* T2;       // the same shifted by one time (T2[... + it - 1]) for pairs that start at odd times
   unsigned long long* counters;  // RTMC_NCOUNTERS or null
   // adjoint
   const float* w;     // dL/dE per bin
@@ -143,6 +144,7 @@ struct LaunchParams {
   // acos(~0) = pi/2, so the scratch holds only that direction bin (BinSetup::tdir)
   int tcomp;
   uint32_t tsp;       // spatial cells per mode in the scratch (time_stride, or stride[3] when tcomp)
+  int odd_pairs;      // pairs also start at odd times (into T2)
   int n_prob;
   unsigned long long* work;  // per-problem particle counters (zeroed)
   unsigned int* active;      // per-problem count of work-groups on it (zeroed)
@@ -902,14 +904,17 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
             } else if (tpend >= 0 && j == tpend + 1) {
-              __hip_atomic_fetch_add((unsigned long long*)P.T + (tpend >> 1), 0x100000001ull, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
+              // the same counter row at two consecutive times: one 64-bit atomic on the
+              // aligned pair, in T from an even time, in T2 (shifted by one) from an odd one
+              unsigned long long* w = (tpend & 1) ? (unsigned long long*)P.T2 + ((tpend - 1) >> 1)
+                                                  : (unsigned long long*)P.T + (tpend >> 1);
+              __hip_atomic_fetch_add(w, 0x100000001ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
             } else {
               if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
               if (j >= 0) {
-                if (!(it & 1) && it + 1 < p.n_times) tpend = j;
+                if (it + 1 < p.n_times && (!(it & 1) || p.odd_pairs)) tpend = j;
                 else __hip_atomic_fetch_add(P.T + j, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               }
             }
