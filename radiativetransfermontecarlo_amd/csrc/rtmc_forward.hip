@@ -249,7 +249,10 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   pl.tsize = tmaj ? (int64_t)p.tsp * pl.n_modes * (int64_t)p.nt_pad : 0;
   // a second scratch shifted by one time takes the pairs that start at odd times
   // when the scratch is small (its clearing and transpose-add are then cheap)
-  p.odd_pairs = tmaj && pl.tsize * 4 <= ((int64_t)2 << 30) && getenv_int("RTMC_ODD_PAIRS", 1) != 0;
+  {
+    const int op = getenv_int("RTMC_ODD_PAIRS", 1);  // 0 off, 1 for scratches <= 2 GiB, 2 always
+    p.odd_pairs = tmaj && (op == 2 || (op == 1 && pl.tsize * 4 <= ((int64_t)2 << 30)));
+  }
   pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
   pl.tmaj_auto = tmaj && hist_mode == 0;
   if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
