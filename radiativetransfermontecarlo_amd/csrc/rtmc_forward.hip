@@ -56,6 +56,53 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
   }
 }
 
+// Runs of three: E[base + sp + it ts + m ms] += sum over the n_sub scratch
+// copies U_g (stride sub_words 64-bit words) and the three shifts s <= it of
+// counter (it - s) % 3 of U_g[((sp + m tsp) * 3 + s) * tw + (it - s) / 3]
+// (21-bit counters at bits 0, 21, 42), through the same 64 x 64 LDS tiles as
+// k_tmajor_add; all copies are summed in one pass, so E is read and written
+// once.
+__global__ __launch_bounds__(256) void k_tmajor_add3(const unsigned long long* __restrict__ U, int n_sub,
+                                                     int64_t sub_words, uint32_t* __restrict__ E, int64_t tsp,
+                                                     const double* __restrict__ dir_edges, int n_dir,
+                                                     int64_t dir_stride, int64_t ts, int64_t ms, int nt, int tw) {
+  __shared__ uint32_t tile[64][65];
+  int64_t base = 0;
+  if (dir_edges) {
+    const int64_t k = bin_search(dir_edges, n_dir, acos(fmin(1.0, fmax(-1.0, cos(kPi / 2)))));
+    if (k < 0) return;  // the tracing kernel deposited nothing in the scratch
+    base = k * dir_stride;
+  }
+  const int64_t sp0 = (int64_t)blockIdx.x * 64;
+  const int it0 = blockIdx.y * 64;
+  const int64_t m = blockIdx.z;
+  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+    const int r = k >> 6, c = k & 63;
+    const int64_t sp = sp0 + r;
+    const int it = it0 + c;
+    uint32_t v = 0u;
+    if (sp < tsp && it < nt) {
+      const size_t row = (size_t)(sp + m * tsp) * 3u;
+      for (int s = 0; s < 3 && s <= it; ++s) {
+        const int q = it - s;
+        const int w = q / 3, f = q - 3 * w;
+        const size_t off = (row + (size_t)s) * (size_t)tw + (size_t)w;
+        for (int g = 0; g < n_sub; ++g)
+          v += (uint32_t)((U[(size_t)g * (size_t)sub_words + off] >> (21 * f)) & 0x1FFFFFull);
+      }
+    }
+    tile[r][c] = v;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+    const int c = k >> 6, r = k & 63;
+    const int64_t sp = sp0 + r;
+    const int it = it0 + c;
+    const uint32_t v = tile[r][c];
+    if (v && sp < tsp && it < nt) E[base + sp + (int64_t)it * ts + m * ms] += v;
+  }
+}
+
 // Global guide of every inverse-CDF table of the launch: cell c of table t of
 // problem p gets the largest knot index k <= n-2 with x[k] <= c/G (0 when
 // c/G < x[0]), by a binary search per cell (Chen and Asau's guide table, here
@@ -106,7 +153,12 @@ struct Plan {
   int64_t tsize;  // HM_TMAJOR: words of one problem's time-major scratch
   bool tmaj_auto; // HM_TMAJOR chosen by default (falls back to HM_GLOBAL without memory)
   bool global_guide;  // build the global guide tables (else the LDS guide, if any)
+  int64_t run_particles;  // runs of three: particles per scratch copy (<= kRunMax)
+  int64_t sub_words;      // runs of three: 64-bit words of one scratch copy
 };
+
+// a 21-bit counter of the runs-of-three scratch takes at most one count per particle
+constexpr int64_t kRunMax = ((int64_t)1 << 21) - 1;
 
 int check_shared(const rtmc_geometry& g, const rtmc_bins& B, int n_times, int time_steps, int precision) {
   int e = check_geometry(&g);
@@ -253,6 +305,23 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
     const int op = getenv_int("RTMC_ODD_PAIRS", 1);  // 0 off, 1 for scratches <= 2 GiB, 2 always
     p.odd_pairs = tmaj && (op == 2 || (op == 1 && pl.tsize * 4 <= ((int64_t)2 << 30)));
   }
+  // runs of three (21-bit counters, one scratch copy per run_particles
+  // particles) when all the copies fit in 2 GiB
+  {
+    const int r3 = getenv_int("RTMC_RUNS3", 1);  // 0 off, 1 when <= 2 GiB, 2 always
+    const int64_t rp = getenv_int("RTMC_RUN_PARTICLES", (int)kRunMax);
+    RTMC_REQUIRE(rp >= 1 && rp <= kRunMax, "RTMC_RUN_PARTICLES must be in [1, %lld]", (long long)kRunMax);
+    pl.run_particles = rp;
+    p.tw = (n_times + 2) / 3;
+    const int64_t rows = (int64_t)p.tsp * pl.n_modes;
+    pl.sub_words = rows * 3 * (int64_t)p.tw;
+    const int64_t nsub = n_total / rp + n_prob;  // scratch copies, at most
+    p.runs3 = tmaj && B.direction_variable == RTMC_DIRVAR_DIRZ && (int64_t)rows * p.nt_pad < ((int64_t)1 << 32) &&
+              (r3 == 2 || (r3 == 1 && nsub * pl.sub_words * 8 <= ((int64_t)2 << 30)));
+    if (p.runs3) p.odd_pairs = 0;
+    p.run_n = (int)rp;
+    p.run_words = pl.sub_words;
+  }
   pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
   pl.tmaj_auto = tmaj && hist_mode == 0;
   if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
@@ -372,8 +441,16 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   const size_t abytes = sizeof(unsigned int) * (size_t)np;
   void* buf = nullptr;
   void* tbuf = nullptr;
-  const size_t tper = (size_t)pl.tsize * (pl.p.odd_pairs ? 2 : 1);  // words per problem: T (and T2)
-  const size_t tbytes = sizeof(uint32_t) * tper * (size_t)np;
+  // scratch of problem i at 32-bit word toff[i]: T (and T2), or with runs of three
+  // one copy of 64-bit words per run_particles particles
+  std::vector<int> nsub(np, 1);
+  std::vector<size_t> toff(np + 1, 0);
+  for (int i = 0; i < np; ++i) {
+    if (pl.p.runs3) nsub[i] = (int)((probs[i].n + pl.run_particles - 1) / pl.run_particles);
+    toff[i + 1] = toff[i] + (pl.p.runs3 ? 2 * (size_t)pl.sub_words * (size_t)nsub[i]
+                                        : (size_t)pl.tsize * (pl.p.odd_pairs ? 2 : 1));
+  }
+  const size_t tbytes = sizeof(uint32_t) * toff[np];
   if (pl.hm == HM_TMAJOR) {
     int e = keep_pool_memory();
     if (e) return e;
@@ -389,9 +466,10 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       pl.p.hot_slots = 0;
       pl.p.tcomp = 0;
       pl.p.odd_pairs = 0;
+      pl.p.runs3 = 0;
     } else {
       for (int i = 0; i < np; ++i) {
-        probs[i].T = (uint32_t*)tbuf + tper * (size_t)i;
+        probs[i].T = (uint32_t*)tbuf + toff[i];
         probs[i].T2 = pl.p.odd_pairs ? probs[i].T + pl.tsize : nullptr;
       }
     }
@@ -454,20 +532,35 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   }
   if (rc == RTMC_OK) {
     timer_mark(st, 0);
+    const int hm = (pl.hm == HM_TMAJOR && pl.p.runs3) ? HM_TMAJOR3 : pl.hm;
     if (pl.dim == 2) {
-      rc = pl.fp32 ? launch_trace<2, float>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+      rc = pl.fp32 ? launch_trace<2, float>(pl.p, dprobs, hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                             pl.block, st)
-                   : launch_trace<2, double>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+                   : launch_trace<2, double>(pl.p, dprobs, hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                              pl.block, st);
     } else {
-      rc = pl.fp32 ? launch_trace<3, float>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+      rc = pl.fp32 ? launch_trace<3, float>(pl.p, dprobs, hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                             pl.block, st)
-                   : launch_trace<3, double>(pl.p, dprobs, pl.hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
+                   : launch_trace<3, double>(pl.p, dprobs, hm, pl.quant, pl.ac, pl.bnd, sink, pl.smem, pl.grid,
                                              pl.block, st);
     }
   }
   if (rc == RTMC_OK) timer_mark(st, 1);
-  if (rc == RTMC_OK && tbuf) {
+  if (rc == RTMC_OK && tbuf && pl.p.runs3) {
+    const int64_t ts = pl.p.time_stride;
+    dim3 grid((unsigned)(((int64_t)pl.p.tsp + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
+    for (int i = 0; i < np && rc == RTMC_OK; ++i) {
+      hipLaunchKernelGGL(k_tmajor_add3, grid, dim3(256), 0, st, (const unsigned long long*)probs[i].T, nsub[i],
+                         pl.sub_words, probs[i].E, (int64_t)pl.p.tsp,
+                         pl.p.tcomp ? pl.p.edges[3] : (const double*)nullptr, pl.p.ne[3], (int64_t)pl.p.stride[3], ts,
+                         pl.p.mode_stride, pl.p.n_times, pl.p.tw);
+      hipError_t le = hipGetLastError();
+      if (le != hipSuccess) {
+        set_error("k_tmajor_add3 launch failed: %s", hipGetErrorString(le));
+        rc = RTMC_ERR_HIP;
+      }
+    }
+  } else if (rc == RTMC_OK && tbuf) {
     const int64_t ts = pl.p.time_stride;
     dim3 grid((unsigned)(((int64_t)pl.p.tsp + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
     for (int i = 0; i < np && rc == RTMC_OK; ++i) {

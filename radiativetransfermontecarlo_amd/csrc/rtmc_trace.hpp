@@ -50,6 +50,14 @@
 //            direct-mapped table of (T index, count) slots, each claimed by the
 //            first index that maps to it and flushed when the work-group leaves
 //            the problem.
+//            Runs of three (LaunchParams::runs3): the scratch holds 64-bit
+//            words of three 21-bit counters for three consecutive times, in
+//            three copies shifted by one time, so that a run of up to three
+//            consecutive deposits into one bin is one atomic whatever time it
+//            starts at.  A 21-bit counter cannot overflow: particle id goes to
+//            scratch copy id / run_n (run_n <= 2^21 - 1), a particle adds at
+//            most one count to a counter, and the hot-counter cache, whose
+//            counts are larger, is flushed straight into E.
 //
 // Adjoint: the same kernel replays every trajectory from its Philox counter
 // (same draws, same events, same deposits) and, at each deposit into bin b
@@ -94,6 +102,9 @@ struct ProbDev {
   uint32_t* E;        // histogram (forward)
   uint32_t* T;        // HM_TMAJOR scratch: T[(spatial + mode * time_stride) * nt_pad + it]
   uint32_t* T2;       // the same shifted by one time (T2[... + it - 1]) for pairs that start at odd times
+                      // runs of three: T holds copies c = 0, 1, ... (run_words 64-bit words
+                      // each) of U[(row * 3 + s) * tw + w], row = spatial + mode * tsp, whose
+                      // word covers times 3w + s + {0, 1, 2} in bits 0-20, 21-41, 42-62
   unsigned long long* counters;  // RTMC_NCOUNTERS or null
   // adjoint
   const float* w;     // dL/dE per bin
@@ -112,7 +123,8 @@ struct ProbDev {
   const int* gg;
 };
 
-enum { HM_GLOBAL = 0, HM_LDS = 1, HM_TMAJOR = 2 };
+// HM_TMAJOR3: HM_TMAJOR with runs of three (LaunchParams::runs3), its own instances
+enum { HM_GLOBAL = 0, HM_LDS = 1, HM_TMAJOR = 2, HM_TMAJOR3 = 3 };
 
 struct LaunchParams {
   rtmc_geometry G;
@@ -145,6 +157,10 @@ struct LaunchParams {
   int tcomp;
   uint32_t tsp;       // spatial cells per mode in the scratch (time_stride, or stride[3] when tcomp)
   int odd_pairs;      // pairs also start at odd times (into T2)
+  int runs3;          // HM_TMAJOR: runs of up to three times per atomic (T holds 21-bit counters)
+  int tw;             // runs of three: words per row and shift, ceil(n_times / 3)
+  int run_n;          // runs of three: particles per scratch copy (<= 2^21 - 1)
+  int64_t run_words;  // runs of three: 64-bit words of one scratch copy
   int n_prob;
   unsigned long long* work;  // per-problem particle counters (zeroed)
   unsigned int* active;      // per-problem count of work-groups on it (zeroed)
@@ -520,7 +536,8 @@ template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK, 
 __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const LaunchParams p,
                                                                      const ProbDev* __restrict__ probs) {
   constexpr bool LDSH = HM == HM_LDS;
-  constexpr bool TMAJ = HM == HM_TMAJOR;
+  constexpr bool TMAJ = HM == HM_TMAJOR || HM == HM_TMAJOR3;
+  constexpr bool RUNS3 = HM == HM_TMAJOR3;
   extern __shared__ double smem[];
   double* sedges = smem;
   __shared__ BinSetup s_bs;
@@ -657,6 +674,19 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     bool has = false;               // lane holds a particle
     bool need = true;               // lane wants a particle
     int64_t tpend = -1;             // HM_TMAJOR: T index of a deposit waiting for its pair
+    // HM_TMAJOR runs of three: the open run's scratch row (-1: none), first time,
+    // length and scratch copy; the scratch copy of the lane's particle
+    int jrow = -1, trun = 0, lrun = 0, rsub = 0, psub = 0;
+    // one atomic for the open run: the word shifted to its first time
+    auto flush_run = [&]() {
+      const unsigned tr = (unsigned)trun;
+      const unsigned w = tr / 3u, s = tr - 3u * w;
+      unsigned long long* a = (unsigned long long*)P.T + (size_t)(uint32_t)rsub * (size_t)p.run_words +
+                              ((size_t)(uint32_t)jrow * 3u + s) * (size_t)(uint32_t)p.tw + w;
+      const unsigned long long inc = lrun == 1 ? 1ull : (lrun == 2 ? 0x200001ull : 0x40000200001ull);
+      __hip_atomic_fetch_add(a, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      jrow = -1;
+    };
     int dbin = 0;                   // bin of the direction variable for the current direction
     int nsc = 0;                    // adjoint: scatterings of the particle so far
     double gsum = 0.0;              // adjoint: sum of dlogsigma_dlc(theta) - dlogSigma_dlc over them
@@ -717,6 +747,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             else if (r - L < nb1 - nb0) id = nb0 + (r - L);
             if (id >= 0) {
               rng.start(P.pid0 + (uint64_t)id);
+              if (RUNS3) psub = (int)(id / p.run_n);
               birth<DIM>(P.S, rng, xb, d, q);
               dbin = dir_bin(p, bs, sedges, DIM, d[2]);
               isP = (!AC && P.S.polarization == 1) ? 0 : 1;
@@ -824,6 +855,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                 __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 tpend = -1;
               }
+              if (RUNS3 && jrow >= 0) flush_run();
             }
           } else {
             // event-driven: scattering at time ts (propagateParticle.m:24-72)
@@ -900,7 +932,24 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                 }
               }
             }
-            if (cached) {
+            if (RUNS3) {
+              // the open run grows while the lane stays in its row at the next time
+              const int jr = j < 0 ? -1 : (int)((uint32_t)sp - tbase + (isP ? 0u : p.tsp));
+              if (cached) {
+                if (jrow >= 0) flush_run();
+              } else if (jrow >= 0 && jr == jrow && it == trun + lrun) {
+                lrun += 1;
+                if (lrun == 3) flush_run();
+              } else {
+                if (jrow >= 0) flush_run();
+                if (jr >= 0) {
+                  jrow = jr;
+                  trun = it;
+                  lrun = 1;
+                  rsub = psub;
+                }
+              }
+            } else if (cached) {
               if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
             } else if (tpend >= 0 && j == tpend + 1) {
@@ -957,12 +1006,26 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     }
 
     if (TMAJ && tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (RUNS3 && jrow >= 0) flush_run();
     __syncthreads();
     if (TMAJ && !ADJ) {
       for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) {
         const unsigned long long v = hslot[k];
         const uint32_t c = (uint32_t)(v & 0xFFFFFFull);
-        if (v != kEmpty && c) __hip_atomic_fetch_add(P.T + (v >> 24), c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v == kEmpty || !c) continue;
+        if (RUNS3) {
+          // straight into E (a slot counts up to 2^24 - 1, more than a 21-bit counter
+          // holds): T index row * nt_pad + t (< 2^32, the host keeps these scratches
+          // small), row = spatial - tbase + mode * tsp
+          const uint32_t j = (uint32_t)(v >> 24);
+          const uint32_t row = j / (uint32_t)p.nt_pad, t = j - row * (uint32_t)p.nt_pad;
+          const uint32_t mode = row >= p.tsp ? 1u : 0u;
+          const uint32_t sp = row - mode * p.tsp + (p.tcomp ? (uint32_t)bs.tdir * p.stride[3] : 0u);
+          __hip_atomic_fetch_add(P.E + ((int64_t)sp + (int64_t)t * p.time_stride + (mode ? p.mode_stride : 0)), c,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          __hip_atomic_fetch_add(P.T + (v >> 24), c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
     if (LDSH && !ADJ) {

@@ -54,6 +54,11 @@ static int launch_dirvar(const LaunchParams& p, const ProbDev* probs, int hm, bo
   if (hm == HM_LDS) return launch_proc<DIM, HM_LDS, R, 0, POS>(p, probs, quant, ac, bnd, smem, grid, block, st);
   if (hm == HM_TMAJOR)
     return launch_proc<DIM, HM_TMAJOR, R, 0, POS>(p, probs, quant, ac, bnd, smem, grid, block, st);
+  if constexpr (!POS) {  // runs of three are built for dir_z only
+    if (hm == HM_TMAJOR3)
+      return launch_proc<DIM, HM_TMAJOR3, R, 0, POS>(p, probs, quant, ac, bnd, smem, grid, block, st);
+  }
+  RTMC_REQUIRE(hm != HM_TMAJOR3, "runs of three are built for the dir_z direction variable only");
   return launch_proc<DIM, HM_GLOBAL, R, 0, POS>(p, probs, quant, ac, bnd, smem, grid, block, st);
 }
 

@@ -438,3 +438,40 @@ def test_hot_cache_work_group_repick_keeps_counts(gpu, monkeypatch):
     E, c, _k = solver.forward_histogram(geometry, source, m, O, 400_000, seed=31, device='cuda', hist_mode=3)
     assert int(ref.sum()) > 0
     assert torch.equal(E, ref) and torch.equal(c, rc)
+
+
+@pytest.mark.parametrize("tmax,run_particles,hot,one_dir", [
+    (0.8, "1000", "4096", "1"),     # 17 times, 101 scratch copies
+    (0.75, "777", "0", "1"),        # 16 times, no hot-counter cache
+    (0.85, "2097151", "0", "0"),    # 18 times, one copy, every direction bin in the scratch
+])
+def test_runs_of_three_match_pairs_and_direct_atomics(gpu, monkeypatch, tmax, run_particles, hot, one_dir):
+    """Runs of up to three consecutive times per 64-bit atomic (21-bit counters,
+    three layouts shifted by one time, one scratch copy per RTMC_RUN_PARTICLES
+    particles, the hot-counter cache flushed into E): the histogram and counters
+    equal direct global atomics and the pairs of 32-bit counters, for every
+    n_times mod 3, with and without the hot-counter cache."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0], 'direction': 'upper'}
+    e = np.linspace(-3, 3, 17)
+    observation = {'x': e, 'y': e, 'directions': np.linspace(0, np.pi, 5), 'time': np.arange(0, tmax + 1e-9, 0.05)}
+    m = cases.elastic_iso(2)
+    m.Q = [40.0, 20.0]
+    m.Frequency = 1.0
+    m = MaterialClass.prepareSigma(m, 2)
+    O = initializeObservation(geometry, False, observation, 1, allow_nd=True)
+    n = 100_003
+    ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=1)
+    assert int(ref.sum()) > 0
+    monkeypatch.setenv("RTMC_HOT_SLOTS", hot)
+    monkeypatch.setenv("RTMC_TMAJOR_ONE_DIRECTION", one_dir)
+    monkeypatch.setenv("RTMC_RUNS3", "0")
+    Ep, cp, _k = solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3)
+    monkeypatch.setenv("RTMC_RUNS3", "2")
+    monkeypatch.setenv("RTMC_RUN_PARTICLES", run_particles)
+    E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3)
+    assert torch.equal(Ep, ref) and torch.equal(cp, rc)
+    assert torch.equal(E3, ref) and torch.equal(c3, rc)
+    # accumulation: a second call adds
+    solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3, E=E3)
+    assert torch.equal(E3, 2 * ref)
