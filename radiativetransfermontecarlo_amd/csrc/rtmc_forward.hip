@@ -59,9 +59,12 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
 // Runs of three: E[base + sp + it ts + m ms] += sum over the n_sub scratch
 // copies U_g (stride sub_words 64-bit words) and the three shifts s <= it of
 // counter (it - s) % 3 of U_g[((sp + m tsp) * 3 + s) * tw + (it - s) / 3]
-// (21-bit counters at bits 0, 21, 42), through the same 64 x 64 LDS tiles as
-// k_tmajor_add; all copies are summed in one pass, so E is read and written
-// once.
+// (21-bit counters at bits 0, 21, 42).  A 64 x 64 tile of (bin, time) counts
+// is summed in LDS in three passes, one per shift: within a pass the words of
+// a row cover disjoint times, so each word of every copy is read once (along
+// the row, coalesced) and added without atomics; the tile then goes out along
+// the bins as in k_tmajor_add (zero counts skipped).  All copies are summed in
+// one pass, so E is read and written once.
 __global__ __launch_bounds__(256) void k_tmajor_add3(const unsigned long long* __restrict__ U, int n_sub,
                                                      int64_t sub_words, uint32_t* __restrict__ E, int64_t tsp,
                                                      const double* __restrict__ dir_edges, int n_dir,
@@ -75,23 +78,33 @@ __global__ __launch_bounds__(256) void k_tmajor_add3(const unsigned long long* _
   }
   const int64_t sp0 = (int64_t)blockIdx.x * 64;
   const int it0 = blockIdx.y * 64;
+  const int it1 = min(it0 + 64, nt);  // times [it0, it1) of this tile
   const int64_t m = blockIdx.z;
-  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
-    const int r = k >> 6, c = k & 63;
-    const int64_t sp = sp0 + r;
-    const int it = it0 + c;
-    uint32_t v = 0u;
-    if (sp < tsp && it < nt) {
-      const size_t row = (size_t)(sp + m * tsp) * 3u;
-      for (int s = 0; s < 3 && s <= it; ++s) {
-        const int q = it - s;
-        const int w = q / 3, f = q - 3 * w;
-        const size_t off = (row + (size_t)s) * (size_t)tw + (size_t)w;
-        for (int g = 0; g < n_sub; ++g)
-          v += (uint32_t)((U[(size_t)g * (size_t)sub_words + off] >> (21 * f)) & 0x1FFFFFull);
+  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) tile[k >> 6][k & 63] = 0u;
+  for (int s = 0; s < 3; ++s) {
+    __syncthreads();
+    if (it1 - 1 < s) continue;  // no time of the tile is covered by this shift
+    // words w with 3w + s + {0, 1, 2} meeting [it0, it1)
+    const int w0 = it0 > s ? (it0 - s) / 3 : 0;
+    const int w1 = (it1 - 1 - s) / 3;
+    const int nw = w1 - w0 + 1;
+    for (int k = threadIdx.x; k < 64 * nw; k += blockDim.x) {
+      const int r = k / nw, w = w0 + (k - r * nw);
+      const int64_t sp = sp0 + r;
+      if (sp >= tsp) continue;
+      const size_t off = ((size_t)(sp + m * tsp) * 3u + (size_t)s) * (size_t)tw + (size_t)w;
+      uint32_t c0 = 0u, c1 = 0u, c2 = 0u;
+      for (int g = 0; g < n_sub; ++g) {
+        const unsigned long long v = U[(size_t)g * (size_t)sub_words + off];
+        c0 += (uint32_t)(v & 0x1FFFFFull);
+        c1 += (uint32_t)((v >> 21) & 0x1FFFFFull);
+        c2 += (uint32_t)(v >> 42);
       }
+      const int t = 3 * w + s;
+      if (t >= it0 && t < it1) tile[r][t - it0] += c0;
+      if (t + 1 >= it0 && t + 1 < it1) tile[r][t + 1 - it0] += c1;
+      if (t + 2 >= it0 && t + 2 < it1) tile[r][t + 2 - it0] += c2;
     }
-    tile[r][c] = v;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
