@@ -475,3 +475,32 @@ def test_runs_of_three_match_pairs_and_direct_atomics(gpu, monkeypatch, tmax, ru
     # accumulation: a second call adds
     solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3, E=E3)
     assert torch.equal(E3, 2 * ref)
+
+
+def test_batched_time_major_items_have_their_own_scratch_copies(gpu, monkeypatch):
+    """A batch of elastic items in the time-major placement with runs of three:
+    each item gets ceil(N / RTMC_RUN_PARTICLES) scratch copies of its own (items
+    of 1, 3 and 9 copies here, one empty item); every item equals its single
+    direct-atomic solve."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-3, 3, 25)
+    observation = {'x': e, 'y': e, 'directions': np.linspace(0, np.pi, 5), 'time': np.arange(0, 0.7 + 1e-9, 0.05)}
+    mats = []
+    for Q in ([40.0, 20.0], [math.inf, math.inf], [80.0, 30.0], [math.inf, math.inf]):
+        m = cases.elastic_iso(2)
+        m.Q = Q
+        m.Frequency = 1.0
+        mats.append(MaterialClass.prepareSigma(m, 2))
+    O = initializeObservation(geometry, False, observation, 1, allow_nd=True)
+    Ns, seeds = [4_000, 0, 11_111, 36_000], [5, 6, 7, 8]
+    refs = [solver.forward_histogram(geometry, source, m, O, n, seed=s, device='cuda', hist_mode=1)[:2]
+            for m, n, s in zip(mats, Ns, seeds)]
+    monkeypatch.setenv("RTMC_RUNS3", "2")
+    monkeypatch.setenv("RTMC_RUN_PARTICLES", "4000")
+    B = rb.Batch(geometry, O, mats, source, Ns, seeds, device='cuda')
+    B.configure(hist_mode=3)
+    E, cnt = B.forward()
+    for i, (Es, cs) in enumerate(refs):
+        assert torch.equal(E[i], Es) and torch.equal(cnt[i], cs), f"item {i}"
+    assert int(E[3].sum()) > 0
