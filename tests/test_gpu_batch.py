@@ -504,3 +504,22 @@ def test_batched_time_major_items_have_their_own_scratch_copies(gpu, monkeypatch
     for i, (Es, cs) in enumerate(refs):
         assert torch.equal(E[i], Es) and torch.equal(cnt[i], cs), f"item {i}"
     assert int(E[3].sum()) > 0
+
+
+@pytest.mark.parametrize("time_steps", [0, 1])
+def test_runs_of_three_acoustic_3d_both_processes(gpu, monkeypatch, time_steps):
+    """Runs of three in the 3-D acoustic instances, sub-stepped (with Q removal)
+    and event-driven (timeSteps = 1): equal to direct global atomics."""
+    geometry, source, base, observation = cases.config1()
+    observation = dict(observation, time=np.arange(0, 1.0 + 1e-9, 0.02))
+    m = rb.prepare_material(base, 3, c=0.08, lc=0.2, Q=300.0)
+    O = initializeObservation(geometry, True, observation, 1)
+    n = 20_000
+    ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, seed=41, device='cuda', hist_mode=1,
+                                           time_steps=time_steps)
+    monkeypatch.setenv("RTMC_RUNS3", "2")
+    monkeypatch.setenv("RTMC_RUN_PARTICLES", "3000")
+    E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, seed=41, device='cuda', hist_mode=3,
+                                          time_steps=time_steps)
+    assert int(ref.sum()) > 0
+    assert torch.equal(E3, ref) and torch.equal(c3, rc)
