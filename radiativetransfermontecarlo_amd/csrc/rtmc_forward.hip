@@ -356,8 +356,8 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   if (tmaj) {
     hot = dim == 2 ? 4096 : 2048;
     if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
-    RTMC_REQUIRE(hot >= 0 && hot <= (1 << 13) && (hot & (hot - 1)) == 0,
-                 "RTMC_HOT_SLOTS must be 0 or a power of two <= 8192");
+    RTMC_REQUIRE(hot == 0 || (hot >= 2 && hot <= (1 << 13) && (hot & (hot - 1)) == 0),
+                 "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]");
     const size_t budget = 40 * 1024;
     while (guide > 256 && edge_bytes + guide_bytes(guide) + 8 + 8 * (size_t)hot > budget) guide >>= 1;
   }
