@@ -683,8 +683,14 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
       const unsigned w = tr / 3u, s = tr - 3u * w;
       unsigned long long* a = (unsigned long long*)P.T + (size_t)(uint32_t)rsub * (size_t)p.run_words +
                               ((size_t)(uint32_t)jrow * 3u + s) * (size_t)(uint32_t)p.tw + w;
-      const unsigned long long inc = lrun == 1 ? 1ull : (lrun == 2 ? 0x200001ull : 0x40000200001ull);
-      __hip_atomic_fetch_add(a, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lrun == 1) {
+        // the first counter (bits 0-20) alone: a 32-bit atomic on the low half, which
+        // the memory side executes faster than a 64-bit one (no carry leaves bit 20)
+        __hip_atomic_fetch_add((uint32_t*)a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_fetch_add(a, lrun == 2 ? 0x200001ull : 0x40000200001ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      }
       jrow = -1;
     };
     int dbin = 0;                   // bin of the direction variable for the current direction
