@@ -1,5 +1,6 @@
 // rtmc_trace_launch.hpp -- compile-time specialisation of k_trace: histogram
-// placement, process, acoustic/elastic, with/without boundaries (the
+// placement (LDS, direct atomics, time-major scratch with pairs or with runs
+// of three), process, acoustic/elastic, with/without boundaries (the
 // event-driven process has none), forward or adjoint (acoustic only), and
 // the direction variable (dir_z or the angle to the position vector).
 // Included by one translation unit per dimension and flight precision.
