@@ -246,9 +246,11 @@ int make_prob(const rtmc_material& m, const rtmc_source& s, int64_t n, uint64_t 
 }
 
 // Shared setup: edges, LDS budget, histogram placement, grid.
+// ev_per_obs: events (scatterings and removals) per observation interval of a
+// particle, weighted over the launch's particles; it sets the default event batch.
 int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt, int time_steps, int precision,
               int acoustics, int block_size, int ppt, int hist_mode, int event_batch, int64_t n_total, int n_prob,
-              bool adjoint, Plan* out) {
+              bool adjoint, double ev_per_obs, Plan* out) {
   Plan& pl = *out;
   LaunchParams& p = pl.p;
   memset((void*)&p, 0, sizeof(p));
@@ -395,7 +397,13 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   pl.block = block;
 
   p.chunk = 256;
-  p.batch = event_batch > 0 ? event_batch : 24;
+  // Lanes waiting for an event before a wavefront takes an event step: more when
+  // events are frequent against observations (an event step then serves more
+  // lanes; deposit steps lose the waiting lanes).  Measured optima: 24 at 0.01
+  // events per observation (3-D README case), 32 at 0.022 (the same with Q = 100)
+  // and 0.05 (2-D elastic, half-space), 48 at 0.35 (lc = 0.05, f = 40, Q = 300:
+  // 1978 ms at 40, 1954 at 48, 2030 at 56 against 2242 at 24).
+  p.batch = event_batch > 0 ? event_batch : (ev_per_obs >= 0.15 ? 48 : (ev_per_obs >= 0.02 ? 32 : 24));
   if (event_batch == 0) {
     if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
   }
@@ -615,6 +623,7 @@ int prepare_batch(const rtmc_batch* bt, const rtmc_adjoint_item* adj, Plan* pl, 
   if (bt->n_items == 0) return RTMC_OK;
   const int ac = bt->items[0].material.acoustics;
   int64_t n_total = 0;
+  double ev_weighted = 0.0;
   probs->clear();
   for (int i = 0; i < bt->n_items; ++i) {
     const rtmc_batch_item& it = bt->items[i];
@@ -656,11 +665,14 @@ int prepare_batch(const rtmc_batch* bt, const rtmc_adjoint_item* adj, Plan* pl, 
     }
     probs->push_back(P);
     n_total += it.n_particles;
+    // scattering and removal tests per observation interval: n_sub (lamS + lamK)
+    ev_weighted += (double)it.n_particles * (double)P.nsub *
+                   (std::max(P.lamS[0], P.lamS[1]) + std::max(P.lamK[0], P.lamK[1]));
   }
   if (probs->empty()) return RTMC_OK;
   e = make_plan(bt->geometry, bt->bins, bt->n_times, bt->dt, bt->time_steps, bt->precision, ac, bt->block_size,
                 bt->particles_per_thread, bt->hist_mode, bt->event_batch, n_total, (int)probs->size(), adj != nullptr,
-                pl);
+                ev_weighted / (double)n_total, pl);
   if (e) return e;
   if (adj) {
     for (auto& P : *probs) P.dhist_stride = pl->hsize;
