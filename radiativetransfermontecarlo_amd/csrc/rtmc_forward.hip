@@ -789,7 +789,7 @@ extern "C" int rtmc_autotune_batched(const rtmc_batch* batch, double sample_frac
   for (int h : hists) {
     std::vector<int> blocks = (h == 2) ? std::vector<int>{1024, 512} : std::vector<int>{256, 128};
     for (int b : blocks)
-      for (int ev : {8, 16, 24, 32})
+      for (int ev : {8, 16, 24, 32, 48})
         for (int ppt : {0, 4}) cands.push_back({b, ppt, h, ev});
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
