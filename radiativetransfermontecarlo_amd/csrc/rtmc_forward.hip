@@ -321,9 +321,9 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
     p.odd_pairs = tmaj && (op == 2 || (op == 1 && pl.tsize * 4 <= ((int64_t)2 << 30)));
   }
   // runs of three (21-bit counters, one scratch copy per run_particles
-  // particles) when all the copies fit in 2 GiB
+  // particles) when all the copies fit in 4 GiB
   {
-    const int r3 = getenv_int("RTMC_RUNS3", 1);  // 0 off, 1 when <= 2 GiB, 2 always
+    const int r3 = getenv_int("RTMC_RUNS3", 1);  // 0 off, 1 when <= 4 GiB, 2 always
     const int64_t rp = getenv_int("RTMC_RUN_PARTICLES", (int)kRunMax);
     RTMC_REQUIRE(rp >= 1 && rp <= kRunMax, "RTMC_RUN_PARTICLES must be in [1, %lld]", (long long)kRunMax);
     pl.run_particles = rp;
@@ -332,7 +332,7 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
     pl.sub_words = rows * 3 * (int64_t)p.tw;
     const int64_t nsub = n_total / rp + n_prob;  // scratch copies, at most
     p.runs3 = tmaj && B.direction_variable == RTMC_DIRVAR_DIRZ && (int64_t)rows * p.nt_pad < ((int64_t)1 << 32) &&
-              (r3 == 2 || (r3 == 1 && nsub * pl.sub_words * 8 <= ((int64_t)2 << 30)));
+              (r3 == 2 || (r3 == 1 && nsub * pl.sub_words * 8 <= ((int64_t)4 << 30)));
     if (p.runs3) p.odd_pairs = 0;
     p.run_n = (int)rp;
     p.run_words = pl.sub_words;
@@ -351,12 +351,15 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   if (pl.global_guide) guide = 0;
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
-  // time-major mode: a cache of hot counters, 8 B a slot.  2-D particles share a
-  // plane, so counters are hit far more often than in 3-D: 4096 slots there,
-  // 2048 in 3-D; the LDS guide, if any, shrinks to keep the work-group within 40 KB.
+  // time-major mode: a cache of hot counters, 8 B a slot: 4096 slots (32 KB), or
+  // 2048 with FP32 flight, whose instances use few registers and would lose half
+  // their work-groups per CU to a larger cache (1e9-particle case: 1.96 s with
+  // 2048, 2.19 s with 4096; FP64 3-D half-space 381.6 -> 375.8 ms with 4096, its
+  // occupancy being set by registers).  The LDS guide, if any, shrinks to keep the
+  // work-group within 40 KB.
   int hot = 0;
   if (tmaj) {
-    hot = dim == 2 ? 4096 : 2048;
+    hot = (dim == 2 || precision != RTMC_PREC_FP32_FLIGHT) ? 4096 : 2048;
     if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
     RTMC_REQUIRE(hot == 0 || (hot >= 2 && hot <= (1 << 13) && (hot & (hot - 1)) == 0),
                  "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]");
