@@ -139,3 +139,26 @@ def test_null_pointers_are_rejected(gpu):
     assert rc != _abi.OK and "must be given" in err
     rc, err = _status(lib.rtmc_forward_batched, None, C.c_void_p(st))
     assert rc != _abi.OK and "batch must be given" in err
+
+
+@pytest.mark.parametrize("var,value,msg", [
+    ("RTMC_HOT_SLOTS", "1", "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]"),
+    ("RTMC_HOT_SLOTS", "3000", "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]"),
+    ("RTMC_RUN_PARTICLES", "0", "RTMC_RUN_PARTICLES must be in [1, 2097151]"),
+    ("RTMC_RUN_PARTICLES", "2097152", "RTMC_RUN_PARTICLES must be in [1, 2097151]"),
+])
+def test_invalid_tuning_variables_are_rejected_before_launch(gpu, monkeypatch, var, value, msg):
+    """The time-major tuning variables are checked when the launch is planned:
+    an error status and message, and E untouched."""
+    geometry = {'dimension': 2, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+    e = np.linspace(-2, 2, 9)
+    observation = {'x': e, 'y': e, 'time': np.arange(0, 0.3 + 1e-9, 0.05)}
+    m = MaterialClass.prepareSigma(cases.elastic_iso(2), 2)
+    O = initializeObservation(geometry, False, observation, 1)
+    monkeypatch.setenv(var, value)
+    E = torch.zeros(O.size, dtype=torch.int32, device='cuda')
+    with pytest.raises(RuntimeError, match=msg.replace("[", r"\[").replace("]", r"\]")):
+        solver.forward_histogram(geometry, source, m, O, 1000, seed=1, device='cuda', hist_mode=3, E=E)
+    torch.cuda.synchronize()
+    assert int(E.abs().sum()) == 0
