@@ -198,10 +198,12 @@ typedef struct {
  * AUTO: LDS when the whole histogram (+ edges) fits in a work-group's LDS,
  * otherwise TMAJOR (GLOBAL if its scratch cannot be allocated).
  * TMAJOR deposits into a time-major scratch copy (time fastest), where a
- * particle that stays in one bin over two consecutive observation times
- * costs one 64-bit atomic instead of two, and adds it into E with a tiled
- * transpose; it holds one scratch histogram per item in the device's default
- * memory pool (kept by the pool for the next call). */
+ * particle that stays in one bin over consecutive observation times costs
+ * one atomic for up to three of them (21-bit counters, one scratch copy per
+ * 2^21 - 1 particles, while all copies fit in 4 GiB) or for two (32-bit
+ * counters, otherwise), and adds it into E with a tiled transpose; the
+ * scratch comes from the device's default memory pool (kept by the pool for
+ * the next call). */
 #define RTMC_HIST_AUTO 0
 #define RTMC_HIST_GLOBAL 1
 #define RTMC_HIST_LDS 2
@@ -223,7 +225,7 @@ typedef struct {
     int32_t block_size;         /* 0 = library default */
     int32_t particles_per_thread; /* 0 = library default */
     int32_t hist_mode;          /* RTMC_HIST_*: 0 = auto, 1 = global atomics, 2 = LDS-privatised,
-                                   3 = time-major scratch with paired 64-bit atomics */
+                                   3 = time-major scratch (runs of up to three or two times per atomic) */
 } rtmc_problem;
 
 /* Information returned by rtmc_propagate_smalldt about its sub-steps. */
