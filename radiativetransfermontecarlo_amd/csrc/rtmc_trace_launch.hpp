@@ -24,7 +24,7 @@ static int launch_one(const LaunchParams& p, const ProbDev* probs, size_t smem, 
 template <int DIM, int HM, typename R, int SINK, bool POS>
 static int launch_proc(const LaunchParams& p, const ProbDev* probs, bool quant, bool ac, bool bnd, size_t smem,
                        unsigned grid, unsigned block, hipStream_t st) {
-  if (SINK == 1) {
+  if constexpr (SINK == 1) {  // the adjoint instances exist for the placements it uses only
     if (quant) {
       if (bnd) return launch_one<DIM, HM, true, true, true, R, 1, POS>(p, probs, smem, grid, block, st);
       return launch_one<DIM, HM, true, true, false, R, 1, POS>(p, probs, smem, grid, block, st);
