@@ -523,3 +523,36 @@ def test_runs_of_three_acoustic_3d_both_processes(gpu, monkeypatch, time_steps):
                                           time_steps=time_steps)
     assert int(ref.sum()) > 0
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
+
+
+@pytest.mark.parametrize("time_steps", [0, 1])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("elastic", [False, True])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_runs_of_three_every_instance_matches_direct_atomics(gpu, monkeypatch, dim, elastic, precision, time_steps):
+    """Every runs-of-three kernel instance (2-D / 3-D, acoustic / elastic, FP64 /
+    FP32 flight, sub-stepped / event-driven) gives the histogram and counters of
+    direct global atomics, with five scratch copies."""
+    geometry = {'dimension': dim, 'frame': 'cartesian'}
+    source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0.0] * dim}
+    e = np.linspace(-2, 2, 17 if dim == 2 else 9)
+    observation = {'x': e, 'y': e, 'time': np.arange(0, 0.6 + 1e-9, 0.05)}
+    if dim == 3:
+        observation['z'] = e
+    if elastic:
+        m = cases.elastic_iso(dim)
+        m.Q = [40.0, 20.0]
+        m.Frequency = 1.0
+    else:
+        m = MaterialClass(geometry, 10, True, 2.0, [0.1, 0.1], 0.0, 'exp', 0.2)
+        m.Q = 100.0
+    m = MaterialClass.prepareSigma(m, dim)
+    O = initializeObservation(geometry, not elastic, observation, 1, allow_nd=True)
+    n = 30_001
+    kw = dict(seed=43, device='cuda', precision=precision, time_steps=time_steps)
+    ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=1, **kw)
+    monkeypatch.setenv("RTMC_RUNS3", "2")
+    monkeypatch.setenv("RTMC_RUN_PARTICLES", "7000")
+    E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=3, **kw)
+    assert int(ref.sum()) > 0
+    assert torch.equal(E3, ref) and torch.equal(c3, rc)
