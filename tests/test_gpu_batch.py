@@ -525,14 +525,17 @@ def test_runs_of_three_acoustic_3d_both_processes(gpu, monkeypatch, time_steps):
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
 
 
+@pytest.mark.parametrize("scratch", ["runs3", "pairs", "even_pairs"])
 @pytest.mark.parametrize("time_steps", [0, 1])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("elastic", [False, True])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_runs_of_three_every_instance_matches_direct_atomics(gpu, monkeypatch, dim, elastic, precision, time_steps):
-    """Every runs-of-three kernel instance (2-D / 3-D, acoustic / elastic, FP64 /
-    FP32 flight, sub-stepped / event-driven) gives the histogram and counters of
-    direct global atomics, with five scratch copies."""
+def test_time_major_every_instance_matches_direct_atomics(gpu, monkeypatch, dim, elastic, precision, time_steps,
+                                                          scratch):
+    """Every time-major kernel instance (2-D / 3-D, acoustic / elastic, FP64 / FP32
+    flight, sub-stepped / event-driven), with runs of three (five scratch copies),
+    pairs from every time or pairs from even times only, gives the histogram and
+    counters of direct global atomics."""
     geometry = {'dimension': dim, 'frame': 'cartesian'}
     source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0.0] * dim}
     e = np.linspace(-2, 2, 17 if dim == 2 else 9)
@@ -551,8 +554,12 @@ def test_runs_of_three_every_instance_matches_direct_atomics(gpu, monkeypatch, d
     n = 30_001
     kw = dict(seed=43, device='cuda', precision=precision, time_steps=time_steps)
     ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=1, **kw)
-    monkeypatch.setenv("RTMC_RUNS3", "2")
-    monkeypatch.setenv("RTMC_RUN_PARTICLES", "7000")
+    if scratch == "runs3":
+        monkeypatch.setenv("RTMC_RUNS3", "2")
+        monkeypatch.setenv("RTMC_RUN_PARTICLES", "7000")
+    else:
+        monkeypatch.setenv("RTMC_RUNS3", "0")
+        monkeypatch.setenv("RTMC_ODD_PAIRS", "2" if scratch == "pairs" else "0")
     E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=3, **kw)
     assert int(ref.sum()) > 0
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
