@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 import torch
 
+from radiativetransfermontecarlo_amd import _abi
 from radiativetransfermontecarlo_amd import batch as rb
 from radiativetransfermontecarlo_amd import solver
 from radiativetransfermontecarlo_amd.material import MaterialClass
@@ -525,18 +526,24 @@ def test_runs_of_three_acoustic_3d_both_processes(gpu, monkeypatch, time_steps):
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
 
 
-@pytest.mark.parametrize("scratch", ["runs3", "pairs", "even_pairs"])
+@pytest.mark.parametrize("placement", ["lds", "runs3", "pairs", "even_pairs"])
+@pytest.mark.parametrize("bnd", [False, True])
 @pytest.mark.parametrize("time_steps", [0, 1])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("elastic", [False, True])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_time_major_every_instance_matches_direct_atomics(gpu, monkeypatch, dim, elastic, precision, time_steps,
-                                                          scratch):
-    """Every time-major kernel instance (2-D / 3-D, acoustic / elastic, FP64 / FP32
-    flight, sub-stepped / event-driven), with runs of three (five scratch copies),
-    pairs from every time or pairs from even times only, gives the histogram and
-    counters of direct global atomics."""
+def test_placements_every_instance_match_direct_atomics(gpu, monkeypatch, dim, elastic, precision, time_steps, bnd,
+                                                        placement):
+    """Every forward kernel instance (2-D / 3-D, acoustic / elastic, FP64 / FP32
+    flight, sub-stepped / event-driven, with or without a reflecting plane) in the
+    LDS histogram and in the time-major scratch with runs of three (five scratch
+    copies), pairs from every time or pairs from even times only gives the
+    histogram and counters of direct global atomics."""
+    if bnd and time_steps == 1:
+        pytest.skip("the event-driven process has no boundaries (propagateParticle.m)")
     geometry = {'dimension': dim, 'frame': 'cartesian'}
+    if bnd:
+        geometry['bnd'] = [{'dir': dim, 'val': 0.6}]   # free surface above the source
     source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0.0] * dim}
     e = np.linspace(-2, 2, 17 if dim == 2 else 9)
     observation = {'x': e, 'y': e, 'time': np.arange(0, 0.6 + 1e-9, 0.05)}
@@ -546,6 +553,7 @@ def test_time_major_every_instance_matches_direct_atomics(gpu, monkeypatch, dim,
         m = cases.elastic_iso(dim)
         m.Q = [40.0, 20.0]
         m.Frequency = 1.0
+        m.rho = 2000.0   # the free surface's conversion tables need it
     else:
         m = MaterialClass(geometry, 10, True, 2.0, [0.1, 0.1], 0.0, 'exp', 0.2)
         m.Q = 100.0
@@ -554,12 +562,14 @@ def test_time_major_every_instance_matches_direct_atomics(gpu, monkeypatch, dim,
     n = 30_001
     kw = dict(seed=43, device='cuda', precision=precision, time_steps=time_steps)
     ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=1, **kw)
-    if scratch == "runs3":
+    if placement == "runs3":
         monkeypatch.setenv("RTMC_RUNS3", "2")
         monkeypatch.setenv("RTMC_RUN_PARTICLES", "7000")
-    else:
+    elif placement != "lds":
         monkeypatch.setenv("RTMC_RUNS3", "0")
-        monkeypatch.setenv("RTMC_ODD_PAIRS", "2" if scratch == "pairs" else "0")
-    E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=3, **kw)
+        monkeypatch.setenv("RTMC_ODD_PAIRS", "2" if placement == "pairs" else "0")
+    E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, hist_mode=2 if placement == "lds" else 3, **kw)
     assert int(ref.sum()) > 0
+    if bnd:
+        assert int(rc[_abi.CNT_BOUNDARY]) > 0
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
