@@ -526,9 +526,12 @@ def test_runs_of_three_acoustic_3d_both_processes(gpu, monkeypatch, time_steps):
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
 
 
+# (time_steps, bnd): the event-driven process has no boundaries (propagateParticle.m)
+_PROCESSES = [(0, False), (0, True), (1, False)]
+
+
 @pytest.mark.parametrize("placement", ["lds", "runs3", "pairs", "even_pairs"])
-@pytest.mark.parametrize("bnd", [False, True])
-@pytest.mark.parametrize("time_steps", [0, 1])
+@pytest.mark.parametrize("time_steps,bnd", _PROCESSES)
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("elastic", [False, True])
 @pytest.mark.parametrize("dim", [2, 3])
@@ -539,8 +542,6 @@ def test_placements_every_instance_match_direct_atomics(gpu, monkeypatch, dim, e
     LDS histogram and in the time-major scratch with runs of three (five scratch
     copies), pairs from every time or pairs from even times only gives the
     histogram and counters of direct global atomics."""
-    if bnd and time_steps == 1:
-        pytest.skip("the event-driven process has no boundaries (propagateParticle.m)")
     geometry = {'dimension': dim, 'frame': 'cartesian'}
     if bnd:
         geometry['bnd'] = [{'dir': dim, 'val': 0.6}]   # free surface above the source
@@ -575,8 +576,7 @@ def test_placements_every_instance_match_direct_atomics(gpu, monkeypatch, dim, e
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
 
 
-@pytest.mark.parametrize("bnd", [False, True])
-@pytest.mark.parametrize("time_steps", [0, 1])
+@pytest.mark.parametrize("time_steps,bnd", _PROCESSES)
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("dim", [2, 3])
 def test_adjoint_every_instance_lds_equals_direct(gpu, dim, precision, time_steps, bnd):
@@ -584,8 +584,6 @@ def test_adjoint_every_instance_lds_equals_direct(gpu, dim, precision, time_step
     event-driven, with or without a reflecting plane): with the weights in LDS or
     read from global memory the replay makes the forward's deposits and
     scatterings (equal counters) and the gradients agree to rounding."""
-    if bnd and time_steps == 1:
-        pytest.skip("the event-driven process has no boundaries (propagateParticle.m)")
     geometry = {'dimension': dim, 'frame': 'cartesian'}
     if bnd:
         geometry['bnd'] = [{'dir': dim, 'val': 0.6}]
