@@ -887,7 +887,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
         // ---------------- DEPOSIT step ----------------
         // Up to DEPS observations per lane (each lane stops at its next event or
         // its last observation): the step's scheduling is shared by them.
-        constexpr int DEPS = TMAJ ? 1 : 4;
+        constexpr int DEPS = TMAJ ? 1 : 8;
 #pragma unroll
         for (int rep = 0; rep < DEPS; ++rep) {
         const bool dep_now =
