@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RTMC_ABI_VERSION 3
+#define RTMC_ABI_VERSION 4
 
 /* status codes */
 #define RTMC_OK 0
@@ -305,6 +305,27 @@ int rtmc_observe(const rtmc_geometry* geometry, const rtmc_bins* bins,
  * propagateParticle.  counters: NULL or device uint64[RTMC_NCOUNTERS]. */
 int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* counters,
                  void* stream);
+
+/* Bytes of device scratch rtmc_forward_noalloc may use per GPU. */
+#define RTMC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* No counterpart in the reference.  rtmc_forward for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, same histogram and
+ * counters as rtmc_forward, and no hipMalloc / hipFree / pool allocation.
+ * The call's device scratch (problem record, particle counters,
+ * inverse-CDF guide, time-major histogram scratch) is carved from a
+ * library-held array of RTMC_NOALLOC_SCRATCH_BYTES on each GPU, which the
+ * GPU keeps until the process ends; its contents are cleared by kernels at
+ * every call.  What does not fit is done without it, as rtmc_forward does
+ * when memory is short (guide: search of the whole table; time-major
+ * scratch: pairs from even times only, then direct atomics into E), except
+ * an explicit hist_mode RTMC_HIST_TMAJOR whose scratch does not fit, which
+ * returns RTMC_ERR_ARG.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams or threads) race and must be
+ * serialised by the caller. */
+int rtmc_forward_noalloc(const rtmc_problem* prob, uint32_t* E, uint64_t* counters,
+                         void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched launches: many materials and sources with their own particle

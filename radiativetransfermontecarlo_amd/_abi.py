@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK = 0
 ERR_ARG = -1
@@ -48,6 +48,14 @@ PREC_FP32_FLIGHT = 1
 DIRVAR_DIRZ = 0
 DIRVAR_POSITION = 1
 DIRVARS = {"dir_z": DIRVAR_DIRZ, "position": DIRVAR_POSITION}
+
+HIST_AUTO = 0
+HIST_GLOBAL = 1
+HIST_LDS = 2
+HIST_TMAJOR = 3
+
+# device scratch of rtmc_forward_noalloc per GPU
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 CNT_SCATTER = 0
 CNT_DEPOSIT = 1
@@ -290,6 +298,7 @@ EXPORTS = (
     "rtmc_propagate_events",
     "rtmc_observe",
     "rtmc_forward",
+    "rtmc_forward_noalloc",
     "rtmc_forward_batched",
     "rtmc_adjoint_batched",
     "rtmc_autotune_batched",
@@ -329,6 +338,8 @@ def _declare(lib):
                                  C.c_int64, C.c_void_p]
     lib.rtmc_forward.restype = C.c_int
     lib.rtmc_forward.argtypes = [P(Problem), C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.rtmc_forward_noalloc.restype = C.c_int
+    lib.rtmc_forward_noalloc.argtypes = [P(Problem), C.c_void_p, C.c_void_p, C.c_void_p]
     lib.rtmc_forward_batched.restype = C.c_int
     lib.rtmc_forward_batched.argtypes = [P(Batch), C.c_void_p]
     lib.rtmc_adjoint_batched.restype = C.c_int

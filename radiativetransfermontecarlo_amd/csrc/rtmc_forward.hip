@@ -10,9 +10,24 @@
 #include <algorithm>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
+// Device scratch of rtmc_forward_noalloc: the problem record, the particle
+// counters, the inverse-CDF guide and the time-major scratch of a call are
+// carved from this array instead of the memory pool.  Each GPU holds its own
+// copy (at its own address, found with hipGetSymbolAddress) until the
+// process ends; calls on one GPU share it, so overlapping calls race.
+static __device__ __align__(256) unsigned char scratch[RTMC_NOALLOC_SCRATCH_BYTES];
+
 namespace rtmc {
+
+// Zero-fill of n 32-bit words (the no-alloc path clears its scratch with a
+// kernel rather than hipMemsetAsync).
+__global__ __launch_bounds__(256) void k_fill_zero(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
 
 // defined in rtmc_trace_{2,3}d_{f64,f32}.hip
 template <int DIM, typename R>
@@ -458,48 +473,84 @@ int keep_pool_memory() {
   return RTMC_OK;
 }
 
-int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
+// Device memory of one launch: from the stream-ordered pool (base null), or
+// carved from the no-alloc scratch array (base = its device address).
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  size_t avail() const {
+    const size_t off = (used + 255) & ~(size_t)255;
+    return off < cap ? cap - off : 0;
+  }
+};
+
+hipError_t arena_alloc(Arena* a, void** p, size_t bytes, hipStream_t st) {
+  if (!a->base) return hipMallocAsync(p, bytes, st);
+  const size_t off = (a->used + 255) & ~(size_t)255;
+  if (off > a->cap || bytes > a->cap - off) return hipErrorOutOfMemory;
+  *p = a->base + off;
+  a->used = off + bytes;
+  return hipSuccess;
+}
+
+hipError_t arena_free(Arena* a, void* p, hipStream_t st) { return a->base ? hipSuccess : hipFreeAsync(p, st); }
+
+// clears bytes (a multiple of 4) at p: hipMemsetAsync, or a fill kernel in the scratch array
+hipError_t arena_zero(Arena* a, void* p, size_t bytes, hipStream_t st) {
+  if (!a->base) return hipMemsetAsync(p, 0, bytes, st);
+  const int64_t n = (int64_t)(bytes / 4);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, st, (uint32_t*)p, n);
+  return hipGetLastError();
+}
+
+// Scratch of the time-major placement per problem (32-bit words at toff[i]): T
+// (and T2), or with runs of three one copy of 64-bit words per run_particles particles.
+size_t tmajor_words(const Plan& pl, const std::vector<ProbDev>& probs, std::vector<int>* nsub,
+                    std::vector<size_t>* toff) {
+  const int np = (int)probs.size();
+  nsub->assign(np, 1);
+  toff->assign(np + 1, 0);
+  for (int i = 0; i < np; ++i) {
+    if (pl.p.runs3) (*nsub)[i] = (int)((probs[i].n + pl.run_particles - 1) / pl.run_particles);
+    (*toff)[i + 1] = (*toff)[i] + (pl.p.runs3 ? 2 * (size_t)pl.sub_words * (size_t)(*nsub)[i]
+                                              : (size_t)pl.tsize * (pl.p.odd_pairs ? 2 : 1));
+  }
+  return (*toff)[np];
+}
+
+// the time-major placement is dropped: direct global atomics need no scratch
+void drop_tmajor(Plan& pl) {
+  pl.hm = HM_GLOBAL;
+  pl.p.hot_slots = 0;
+  pl.p.tcomp = 0;
+  pl.p.odd_pairs = 0;
+  pl.p.runs3 = 0;
+}
+
+int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Arena* arena = nullptr) {
+  Arena pool;  // default: the stream-ordered pool
+  Arena& ar = arena ? *arena : pool;
   const int np = (int)probs.size();
   const size_t pbytes = sizeof(ProbDev) * (size_t)np;
   const size_t wbytes = sizeof(unsigned long long) * (size_t)np;
   const size_t abytes = sizeof(unsigned int) * (size_t)np;
   void* buf = nullptr;
   void* tbuf = nullptr;
-  // scratch of problem i at 32-bit word toff[i]: T (and T2), or with runs of three
-  // one copy of 64-bit words per run_particles particles
-  std::vector<int> nsub(np, 1);
-  std::vector<size_t> toff(np + 1, 0);
-  for (int i = 0; i < np; ++i) {
-    if (pl.p.runs3) nsub[i] = (int)((probs[i].n + pl.run_particles - 1) / pl.run_particles);
-    toff[i + 1] = toff[i] + (pl.p.runs3 ? 2 * (size_t)pl.sub_words * (size_t)nsub[i]
-                                        : (size_t)pl.tsize * (pl.p.odd_pairs ? 2 : 1));
-  }
-  const size_t tbytes = sizeof(uint32_t) * toff[np];
-  if (pl.hm == HM_TMAJOR) {
+  void* gbuf = nullptr;
+  if (!ar.base) {
     int e = keep_pool_memory();
     if (e) return e;
-    hipError_t ae = hipMallocAsync(&tbuf, tbytes, st);
-    if (ae != hipSuccess) {
-      tbuf = nullptr;
-      if (!pl.tmaj_auto) {
-        set_error("time-major scratch of %zu bytes: %s (use hist_mode 1)", tbytes, hipGetErrorString(ae));
-        return RTMC_ERR_NOMEM;
-      }
-      (void)hipGetLastError();
-      pl.hm = HM_GLOBAL;  // direct global atomics need no scratch
-      pl.p.hot_slots = 0;
-      pl.p.tcomp = 0;
-      pl.p.odd_pairs = 0;
-      pl.p.runs3 = 0;
-    } else {
-      for (int i = 0; i < np; ++i) {
-        probs[i].T = (uint32_t*)tbuf + toff[i];
-        probs[i].T2 = pl.p.odd_pairs ? probs[i].T + pl.tsize : nullptr;
-      }
-    }
+  }
+  // problem records and counters first: nothing runs without them
+  if (arena_alloc(&ar, &buf, pbytes + wbytes + abytes, st) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("%s of %zu bytes of problem data failed", ar.base ? "no-alloc scratch" : "hipMallocAsync",
+              pbytes + wbytes + abytes);
+    return RTMC_ERR_NOMEM;
   }
   // global guides of the inverse-CDF tables: about one cell per knot, G a power of two
-  void* gbuf = nullptr;
   int64_t nmax = 2;
   for (int i = 0; i < np; ++i)
     for (int t = 0; t < pl.p.ntab; ++t) nmax = std::max<int64_t>(nmax, probs[i].M.inv[t].n);
@@ -511,7 +562,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   pl.p.gg_log2 = 0;
   const int lds_guide = pl.p.guide_n;  // restored below: the plan may be run again
   if (want_gg) {
-    if (keep_pool_memory() == RTMC_OK && hipMallocAsync(&gbuf, gbytes, st) == hipSuccess) {
+    if (arena_alloc(&ar, &gbuf, gbytes, st) == hipSuccess) {
       for (int i = 0; i < np; ++i) probs[i].gg = (const int*)gbuf + gcells * (size_t)pl.p.ntab * (size_t)i;
       pl.p.gg_log2 = lg;
       pl.p.guide_n = 0;  // the LDS guide is not needed
@@ -520,13 +571,47 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
       (void)hipGetLastError();  // fall back to the LDS guide
     }
   }
-  if (hipMallocAsync(&buf, pbytes + wbytes + abytes, st) != hipSuccess) {
-    pl.p.guide_n = lds_guide;
-    pl.p.gg_log2 = 0;
-    if (gbuf) (void)hipFreeAsync(gbuf, st);
-    if (tbuf) (void)hipFreeAsync(tbuf, st);
-    set_error("hipMallocAsync of %zu bytes of problem data failed", pbytes + wbytes + abytes);
-    return RTMC_ERR_NOMEM;
+  std::vector<int> nsub;
+  std::vector<size_t> toff;
+  size_t tbytes = 0;
+  if (pl.hm == HM_TMAJOR) {
+    tbytes = sizeof(uint32_t) * tmajor_words(pl, probs, &nsub, &toff);
+    if (ar.base && tbytes > ar.avail()) {
+      // the scratch array is too small for runs of three: pairs, then pairs from even times only
+      if (pl.p.runs3) {
+        pl.p.runs3 = 0;
+        pl.p.odd_pairs = 1;
+        tbytes = sizeof(uint32_t) * tmajor_words(pl, probs, &nsub, &toff);
+      }
+      if (tbytes > ar.avail() && pl.p.odd_pairs) {
+        pl.p.odd_pairs = 0;
+        tbytes = sizeof(uint32_t) * tmajor_words(pl, probs, &nsub, &toff);
+      }
+    }
+    hipError_t ae = arena_alloc(&ar, &tbuf, tbytes, st);
+    if (ae != hipSuccess) {
+      tbuf = nullptr;
+      (void)hipGetLastError();
+      if (!pl.tmaj_auto) {
+        pl.p.guide_n = lds_guide;
+        pl.p.gg_log2 = 0;
+        if (gbuf) (void)arena_free(&ar, gbuf, st);
+        (void)arena_free(&ar, buf, st);
+        if (ar.base) {
+          set_error("time-major scratch of %zu bytes exceeds the %llu bytes of the no-alloc scratch (use hist_mode 0 "
+                    "or 1)", tbytes, (unsigned long long)RTMC_NOALLOC_SCRATCH_BYTES);
+          return RTMC_ERR_ARG;
+        }
+        set_error("time-major scratch of %zu bytes: %s (use hist_mode 1)", tbytes, hipGetErrorString(ae));
+        return RTMC_ERR_NOMEM;
+      }
+      drop_tmajor(pl);
+    } else {
+      for (int i = 0; i < np; ++i) {
+        probs[i].T = (uint32_t*)tbuf + toff[i];
+        probs[i].T2 = pl.p.odd_pairs ? probs[i].T + pl.tsize : nullptr;
+      }
+    }
   }
   char* b = (char*)buf;
   ProbDev* dprobs = (ProbDev*)b;
@@ -534,8 +619,8 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   pl.p.active = (unsigned int*)(b + pbytes + wbytes);
   int rc = RTMC_OK;
   hipError_t e = hipMemcpyAsync(dprobs, probs.data(), pbytes, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemsetAsync(pl.p.work, 0, wbytes + abytes, st);
-  if (e == hipSuccess && tbuf) e = hipMemsetAsync(tbuf, 0, tbytes, st);
+  if (e == hipSuccess) e = arena_zero(&ar, pl.p.work, wbytes + abytes, st);
+  if (e == hipSuccess && tbuf) e = arena_zero(&ar, tbuf, tbytes, st);
   if (e != hipSuccess) {
     set_error("problem upload failed: %s", hipGetErrorString(e));
     rc = RTMC_ERR_HIP;
@@ -603,9 +688,9 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st) {
   }
   pl.p.guide_n = lds_guide;
   pl.p.gg_log2 = 0;
-  hipError_t fe = hipFreeAsync(buf, st);
-  hipError_t fe2 = tbuf ? hipFreeAsync(tbuf, st) : hipSuccess;
-  hipError_t fe3 = gbuf ? hipFreeAsync(gbuf, st) : hipSuccess;
+  hipError_t fe = arena_free(&ar, buf, st);
+  hipError_t fe2 = tbuf ? arena_free(&ar, tbuf, st) : hipSuccess;
+  hipError_t fe3 = gbuf ? arena_free(&ar, gbuf, st) : hipSuccess;
   if (rc == RTMC_OK && (fe != hipSuccess || fe2 != hipSuccess || fe3 != hipSuccess)) {
     set_error("hipFreeAsync failed: %s",
               hipGetErrorString(fe != hipSuccess ? fe : (fe2 != hipSuccess ? fe2 : fe3)));
@@ -715,6 +800,73 @@ extern "C" int rtmc_forward(const rtmc_problem* prob, uint32_t* E, uint64_t* cou
   bt.n_items = 1;
   bt.items = &item;
   return rtmc_forward_batched(&bt, stream);
+}
+
+namespace {
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current (host code passes this address, never the array's name).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  RTMC_HIP(hipGetDevice(&dev));
+  RTMC_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d: the no-alloc scratch serves devices 0..%d", dev,
+               kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+      return RTMC_ERR_HIP;
+    }
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return RTMC_OK;
+}
+
+}  // namespace
+
+extern "C" int rtmc_forward_noalloc(const rtmc_problem* prob, uint32_t* E, uint64_t* counters, void* stream) {
+  clear_error();
+  RTMC_REQUIRE(prob != nullptr && E != nullptr, "problem and E must be given");
+  void* base = nullptr;
+  int e = noalloc_scratch(&base);
+  if (e) return e;
+  rtmc_batch_item item;
+  item.material = prob->material;
+  item.source = prob->source;
+  item.n_particles = prob->n_particles;
+  item.particle_offset = prob->particle_offset;
+  item.seed = prob->seed;
+  item.E = E;
+  item.counters = counters;
+  rtmc_batch bt;
+  memset(&bt, 0, sizeof(bt));
+  bt.geometry = prob->geometry;
+  bt.bins = prob->bins;
+  bt.n_times = prob->n_times;
+  bt.time_steps = prob->time_steps;
+  bt.dt = prob->dt;
+  bt.precision = prob->precision;
+  bt.block_size = prob->block_size;
+  bt.particles_per_thread = prob->particles_per_thread;
+  bt.hist_mode = prob->hist_mode;
+  bt.n_items = 1;
+  bt.items = &item;
+  Plan pl;
+  std::vector<ProbDev> probs;
+  bool empty = true;
+  e = prepare_batch(&bt, nullptr, &pl, &probs, &empty);
+  if (e || empty) return e;
+  Arena ar;
+  ar.base = (char*)base;
+  ar.cap = (size_t)RTMC_NOALLOC_SCRATCH_BYTES;
+  return run_plan(pl, probs, 0, (hipStream_t)stream, &ar);
 }
 
 extern "C" int rtmc_forward_batched(const rtmc_batch* batch, void* stream) {
