@@ -1,0 +1,174 @@
+"""Elastic boundary partition cases of Examples/BoundaryReflectionEnergyTests.m,
+shared by the oracle pins (CPU) and the kernel tests (GPU).
+
+A medium without volume scattering (mean free times Inf, zero inverse CDFs:
+BoundaryReflectionEnergyTests.m:220-249), vp = 2500, vs = vp / sqrt(3),
+rho = 2000 (:30-32), one reflecting plane z = 0 (:52-53) or a cylinder, and
+pure P, SV or SH plane packets at the reference's incidence angles (:41-43).
+After the crossing, the energy fractions E_P, E_SV, E_SH (computeModeEnergies,
+:345-375, with the SV/SH basis of svshBasis, :380-400) must equal what the
+code's decisions imply (expectedBoundaryEnergies, :405-466):
+  P  at theta:  E_P = E_Rpp(theta), E_SV = 1 - E_Rpp(theta)
+  SV at theta:  E_P = 1 - E_Rsvsv(theta), E_SV = E_Rsvsv(theta) below the
+                SV->P critical angle; above it E_P = 0 and E_SV = R + (1 - R)
+                cos^2(2 theta) (the fallback reflects dir alone, see expected())
+  SH at theta:  E_SH = 1
+with E_Rpp and E_Rsvsv the solid/air energy coefficients of
+MaterialClass.Zoeppritz (pinned separately by energy balance).
+"""
+import math
+
+import numpy as np
+
+from radiativetransfermontecarlo_amd import _abi
+from radiativetransfermontecarlo_amd.material import zoeppritz_fluid
+
+VP = 2500.0
+VS = VP / math.sqrt(3.0)
+RHO = 2000.0
+P_ANGLES = (15.0, 30.0, 45.0, 60.0)
+SV_ANGLES = (10.0, 20.0, 30.0, 40.0)     # 40 is beyond the SV->P critical angle (35.26)
+SH_ANGLES = (15.0, 45.0, 60.0)
+CRIT = math.degrees(math.asin(VS / VP))
+
+
+def zoeppritz():
+    j1 = np.linspace(0, 90, 181)
+    out = zoeppritz_fluid(j1, VP, VS, RHO, 343.0, 0.0, 1.0)
+    out['j1_deg'] = j1
+    return out
+
+
+def interp(out, key, deg):
+    return float(np.interp(deg, out['j1_deg'], out[key]))
+
+
+def material_arrays():
+    """Host arrays of the no-scattering elastic material (rtmc_material fields)."""
+    z = zoeppritz()
+    return {'x': np.array([0.0, 1.0]), 'y': np.array([0.0, 0.0]), 'deg': np.ascontiguousarray(z['j1_deg']),
+            'Rpp': np.ascontiguousarray(z['E_Rpp']), 'Rsvsv': np.ascontiguousarray(z['E_Rsvsv'])}
+
+
+def material_struct(arrays, ptr):
+    """rtmc_material over the arrays; ptr(a) gives the address to store."""
+    m = _abi.Material()
+    m.acoustics = 0
+    m.vp, m.vs = VP, VS
+    m.mft[0] = m.mft[1] = math.inf
+    m.P2P = m.S2S = 1.0
+    m.QP = m.QS = math.inf
+    m.frequency = math.nan
+    for k in range(4):
+        m.invcdf[k].n = 2
+        m.invcdf[k].x = ptr(arrays['x'])
+        m.invcdf[k].y = ptr(arrays['y'])
+        m.invcdf[k].extrap = _abi.EXTRAP_LINEAR
+    m.n_zoe = arrays['deg'].size
+    m.zoe_deg, m.zoe_Rpp, m.zoe_Rsvsv = ptr(arrays['deg']), ptr(arrays['Rpp']), ptr(arrays['Rsvsv'])
+    return m
+
+
+def _unit(a):
+    return a / np.linalg.norm(a, axis=0)
+
+
+def svsh_basis(d, n):
+    """(eSV, eSH) of directions d (3 x N) relative to the normals n (3 x N)."""
+    eSH = _unit(np.cross(d.T, n.T).T)
+    eSV = _unit(np.cross(eSH.T, d.T).T)
+    return eSV, eSH
+
+
+def plane_packet(mode, deg, N, rng, dt):
+    """Pure packet moving down onto z = 0 (createIncidentPlanePacket, :253-330):
+    positions on a 20 x 20 aperture centred above the plane such that every
+    particle crosses within the time dt (one sub-step without scattering)."""
+    th = math.radians(deg)
+    d0 = np.array([math.sin(th), 0.0, -math.cos(th)])
+    v = VP if mode == 'P' else VS
+    depth = rng.uniform(0.05, 0.95, N) * v * dt * math.cos(th)   # height above z = 0
+    x = np.zeros((3, N))
+    x[0] = rng.uniform(-10, 10, N)
+    x[1] = rng.uniform(-10, 10, N)
+    x[2] = depth
+    d = np.repeat(d0[:, None], N, axis=1)
+    n = np.repeat(np.array([[0.0], [0.0], [1.0]]), N, axis=1)
+    eSV, eSH = svsh_basis(d, n)
+    perp = eSH if mode in ('P', 'SH') else eSV
+    p = np.full(N, 1 if mode == 'P' else 0, dtype=np.uint8)
+    return x, d, perp.copy(), p
+
+
+def cylinder_packet(mode, deg, N, rng, dt, R=1000.0):
+    """Packet just inside a cylinder of radius R around z, moving outward at
+    `deg` from the radial normal in the (radial, z) plane, crossing within dt."""
+    th = math.radians(deg)
+    phi = rng.uniform(0, 2 * math.pi, N)                         # azimuth of the hit point
+    nr = np.stack([np.cos(phi), np.sin(phi), np.zeros(N)])
+    v = VP if mode == 'P' else VS
+    gap = rng.uniform(0.05, 0.95, N) * v * dt * math.cos(th)
+    x = nr * (R - gap)
+    x[2] = rng.uniform(-5, 5, N)
+    d = nr * math.cos(th)
+    d[2] = math.sin(th)
+    eSV, eSH = svsh_basis(d, nr)
+    perp = eSH if mode in ('P', 'SH') else eSV
+    p = np.full(N, 1 if mode == 'P' else 0, dtype=np.uint8)
+    return x, d, perp.copy(), p
+
+
+def energies(d, perp, p, n):
+    """(E_P, E_SV, E_SH) of computeModeEnergies (:345-375) with normals n."""
+    N = p.size
+    s = p == 0
+    EP = p.mean()
+    if not s.any():
+        return EP, 0.0, 0.0
+    eSV, eSH = svsh_basis(d[:, s], n[:, s])
+    aSV = np.sum(perp[:, s] * eSV, axis=0)
+    aSH = np.sum(perp[:, s] * eSH, axis=0)
+    return EP, float(np.sum(aSV ** 2) / N), float(np.sum(aSH ** 2) / N)
+
+
+def expected(mode, deg, table):
+    """(E_P, E_SV, E_SH) the code's decisions imply (expectedBoundaryEnergies, :405-466)."""
+    if mode == 'P':
+        R = interp(table, 'E_Rpp', deg)
+        return R, 1 - R, 0.0
+    if mode == 'SV':
+        R = interp(table, 'E_Rsvsv', deg)
+        if deg < CRIT:
+            return 1 - R, R, 0.0
+        # beyond the critical angle the attempted SV->P (probability 1 - R) falls
+        # back to reflecting dir alone, perp unflipped (propagateParticleSmallDt.m:
+        # 312-318): such a particle keeps SV weight (perp . eSV')^2 = cos^2(2 theta),
+        # where the reference test expects 1 (:434-438)
+        return 0.0, R + (1 - R) * math.cos(math.radians(2 * deg)) ** 2, 0.0
+    return 0.0, 0.0, 1.0
+
+
+def check_partition(mode, deg, x0, d0, d, perp, p, N, normals, table, sigmas=4.0):
+    """Energy fractions within `sigmas` binomial standard errors of the
+    expectation, and the outgoing directions: mirror for reflections, Snell's
+    law (sin ratio vs/vp or vp/vs, away from the boundary) for conversions."""
+    EP, ESV, ESH = energies(d, perp, p, normals)
+    xP, xSV, xSH = expected(mode, deg, table)
+    for got, want in ((EP, xP), (ESV, xSV), (ESH, xSH)):
+        tol = sigmas * math.sqrt(max(want * (1 - want), 1.0 / N) / N) + 1e-12
+        assert abs(got - want) <= tol, (mode, deg, got, want, tol)
+    dn_in = np.sum(d0 * normals, axis=0)
+    dn_out = np.sum(d * normals, axis=0)
+    assert np.all(dn_out * dn_in < 0), "every particle leaves the boundary"
+    tin = np.linalg.norm(d0 - dn_in * normals, axis=0)
+    tout = np.linalg.norm(d - dn_out * normals, axis=0)
+    p_in = np.full(N, 1 if mode == 'P' else 0)
+    same = p == p_in
+    np.testing.assert_allclose(tout[same], tin[same], rtol=0, atol=1e-12)          # mirror
+    np.testing.assert_allclose(dn_out[same], -dn_in[same], rtol=0, atol=1e-12)
+    conv = ~same
+    if conv.any():
+        ratio = VS / VP if mode == 'P' else VP / VS
+        np.testing.assert_allclose(tout[conv], ratio * tin[conv], rtol=1e-12, atol=1e-12)   # Snell
+    np.testing.assert_allclose(np.linalg.norm(d, axis=0), 1.0, atol=1e-12)
+    return EP, ESV, ESH
