@@ -183,6 +183,46 @@ class HostMaterial:
         self.c = m
 
 
+def host_material_from_medium(medium, Q=math.inf, rho: Optional[float] = None) -> HostMaterial:
+    """rtmc_material over host arrays from the oracle's own scattering model
+    (oracle/sigma.py: Sigma, mean free times, P2P / S2S and inverse-CDF knots
+    restated from MaterialClass.m independently of the product); rho gives the
+    solid/air Zoeppritz tables of an elastic medium with a free surface."""
+    from oracle import sigma as osg
+    from radiativetransfermontecarlo_amd.material import zoeppritz_fluid
+    ref = osg.prepare(medium)
+    hm = HostMaterial.__new__(HostMaterial)
+    hm.keep = []
+    m = _abi.Material()
+    tabs = ref['tables']
+    if medium.acoustics:
+        m.acoustics = 1
+        m.vp = float(medium.v[0])
+        m.mft[0] = m.mft[1] = float(ref['mft'][0])
+        m.P2P = m.S2S = 1.0
+        m.invcdf[0] = _table(tabs[0][0], hm.keep)
+    else:
+        m.acoustics = 0
+        m.vp, m.vs = float(medium.v[0]), float(medium.v[1])
+        m.mft[0], m.mft[1] = float(ref['mft'][0]), float(ref['mft'][1])
+        m.P2P, m.S2S = float(ref['P2P']), float(ref['S2S'])
+        for k, t in enumerate([tabs[0][0], tabs[0][1], tabs[1][0], tabs[1][1]]):
+            m.invcdf[k] = _table(t, hm.keep)
+    Qa = np.atleast_1d(np.asarray(Q, dtype=np.float64))
+    m.QP = Qa[0]
+    m.QS = Qa[1] if (Qa.size > 1 and not medium.acoustics) else Qa[0]
+    m.frequency = float(medium.freq)
+    if rho is not None and not medium.acoustics:
+        j1 = np.linspace(0, 90, 181)
+        z = zoeppritz_fluid(j1, m.vp, m.vs, float(rho), 343.0, 0.0, 1.0)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (j1, z['E_Rpp'], z['E_Rsvsv'])]
+        hm.keep += arrs
+        m.n_zoe = arrs[0].size
+        m.zoe_deg, m.zoe_Rpp, m.zoe_Rsvsv = (_ptr(a) for a in arrs)
+    hm.c = m
+    return hm
+
+
 class HostBins:
     def __init__(self, O):
         self.keep = []
@@ -250,16 +290,19 @@ def observe(geometry_c: _abi.Geometry, bins: HostBins, acoustics: bool, P: HostP
 
 
 def forward(geometry, source, material: MaterialClass, O, N: int, seed: int, time_steps: int = 0,
-            packet: int = 100_000, threads: int = 0, particle_offset: int = 0, return_counters: bool = False):
+            packet: int = 100_000, threads: int = 0, particle_offset: int = 0, return_counters: bool = False,
+            host_material: Optional[HostMaterial] = None):
     """Whole solve on the CPU (radiativeTransfer.m:95-158 restated): returns the
-    flat uint32 histogram and the deposit count."""
+    flat uint32 histogram and the deposit count.  host_material (e.g. from
+    host_material_from_medium) replaces the tables of `material`."""
     from radiativetransfermontecarlo_amd import solver
     d = int(geometry['dimension'])
     keep = []
     prob = _abi.Problem()
     prob.geometry = solver.build_geometry(geometry)
     prob.source = host_source(source, d, keep)
-    hm = HostMaterial(material, with_zoeppritz=prob.geometry.n_bnd > 0)
+    hm = host_material if host_material is not None else HostMaterial(material,
+                                                                     with_zoeppritz=prob.geometry.n_bnd > 0)
     prob.material = hm.c
     hb = HostBins(O)
     prob.bins = hb.c

@@ -64,7 +64,7 @@ def material_struct(arrays, ptr):
         m.invcdf[k].x = ptr(arrays['x'])
         m.invcdf[k].y = ptr(arrays['y'])
         m.invcdf[k].extrap = _abi.EXTRAP_LINEAR
-    m.n_zoe = arrays['deg'].size
+    m.n_zoe = len(arrays['deg'])
     m.zoe_deg, m.zoe_Rpp, m.zoe_Rsvsv = ptr(arrays['deg']), ptr(arrays['Rpp']), ptr(arrays['Rsvsv'])
     return m
 
