@@ -24,6 +24,11 @@ struct U4 {
 __device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  // The key is opaque here, so the key schedule (k + r W) is recomputed by
+  // scalar adds at each call instead of being hoisted out of the callers' loops
+  // as 20 loop-invariant registers (which spill and then cost a v_readlane per
+  // round).  The asm emits no instruction.
+  asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;

@@ -393,7 +393,7 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
     RTMC_REQUIRE(m >= 256 && m <= cap, "RTMC_HOT_MAX_PARTICLES must be in [256, %lld]", (long long)cap);
     p.hot_max_taken = (unsigned)m;
   }
-  p.hot_shift = 64;
+  p.hot_shift = 32;
   for (int v = hot; v > 1; v >>= 1) p.hot_shift -= 1;
   p.hot_off = (int)((pl.smem + 7) & ~(size_t)7);
   if (hot) pl.smem = (size_t)p.hot_off + 8 * (size_t)hot;
@@ -417,15 +417,30 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   p.chunk = 256;
   // Lanes waiting for an event before a wavefront takes an event step: more when
   // events are frequent against observations (an event step then serves more
-  // lanes; deposit steps lose the waiting lanes).  Measured optima: 24 at 0.01
-  // events per observation (3-D README case), 32 at 0.022 (the same with Q = 100)
-  // and 0.05 (2-D elastic, half-space), 48 at 0.35 (lc = 0.05, f = 40, Q = 300:
-  // 1978 ms at 40, 1954 at 48, 2030 at 56 against 2242 at 24).
-  p.batch = event_batch > 0 ? event_batch : (ev_per_obs >= 0.15 ? 48 : (ev_per_obs >= 0.02 ? 32 : 24));
+  // lanes; deposit steps lose the waiting lanes).  Measured optima (event steps
+  // with block-aligned draws and deferred births): 16 at 0.01 events per
+  // observation (3-D README case: 1.46 ms at 24, 1.36 at 16), 32 at 0.05 (2-D
+  // elastic and half-space cases), 40 at 0.35 (lc = 0.05, f = 40, Q = 300:
+  // 193.7 ms per 1e8 particles at 48, 190.3 at 40, 207.7 at 56).
+  // FP32 flight runs twice the wavefronts per SIMD and prefers 48 there (1e9-particle
+  // case per 1e8 particles: 185 ms at 48 against 196 at 40).
+  const bool f32 = precision == RTMC_PREC_FP32_FLIGHT;
+  p.batch = event_batch > 0 ? event_batch
+                            : (ev_per_obs >= 0.15 ? (f32 ? 48 : 40) : (ev_per_obs >= 0.02 ? 32 : 16));
   if (event_batch == 0) {
     if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
   }
   RTMC_REQUIRE(p.batch >= 1 && p.batch <= 64, "event batch must be in [1, 64]");
+  // Births wait for birth_min lanes when events are frequent (the birth path then
+  // runs for several lanes at once): 1e9-particle case (0.35 events per
+  // observation) 270 -> 236 ms per 1e8 particles with 8, the 2-D elastic and
+  // half-space cases (0.05) 1-2 % faster; the README case (0.01, few events,
+  // many births per event step) 7 % slower, so it keeps 1.  FP32 flight, with
+  // twice the wavefronts per SIMD, loses with 8 (202.6 ms against 185.8 with 1)
+  // and gains a little with 2 at its batch of 48 (180.5 ms).
+  p.birth_min = f32 ? (ev_per_obs >= 0.15 ? 2 : 1) : (ev_per_obs >= 0.02 ? 8 : 1);
+  if (const char* b = getenv("RTMC_BIRTH_MIN")) p.birth_min = atoi(b);
+  RTMC_REQUIRE(p.birth_min >= 1 && p.birth_min <= 64, "RTMC_BIRTH_MIN must be in [1, 64]");
   pl.dim = dim;
   pl.quant = time_steps == 0;
   pl.ac = acoustics != 0;
@@ -944,7 +959,7 @@ extern "C" int rtmc_autotune_batched(const rtmc_batch* batch, double sample_frac
   for (int h : hists) {
     std::vector<int> blocks = (h == 2) ? std::vector<int>{1024, 512} : std::vector<int>{256, 128};
     for (int b : blocks)
-      for (int ev : {8, 16, 24, 32, 48})
+      for (int ev : {8, 16, 24, 32, 40, 48})
         for (int ppt : {0, 4}) cands.push_back({b, ppt, h, ev});
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;

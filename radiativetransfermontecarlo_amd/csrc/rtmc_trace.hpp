@@ -142,13 +142,14 @@ struct LaunchParams {
   int edges_total;    // doubles of the LDS edge copy (0: edges read from global memory)
   int chunk;          // particles fetched per counter access (per wavefront)
   int batch;          // lanes waiting for an event before a wavefront takes an event step
+  int birth_min;      // lanes wanting a particle before an event step births them (while others can fly)
   int ntab;           // inverse-CDF tables (1 acoustic, 4 elastic)
   int guide_n;        // LDS guide cells per table (power of two; 0 = none)
   int gg_log2;        // log2 of the global guide's cells per table (0 = none; then the LDS guide)
   int64_t hist_size;  // words of the LDS histogram / weight table (0 = global memory)
   int nt_pad;         // HM_TMAJOR: n_times rounded up to even (time stride of T is 1)
   int hot_slots;      // HM_TMAJOR: slots of the LDS cache of hot counters (power of two, or 0)
-  int hot_shift;      // 64 - log2(hot_slots)
+  int hot_shift;      // 32 - log2(hot_slots)
   int hot_off;        // byte offset of the cache in the dynamic LDS block
   int hot_times;      // only deposits at observation indices below this go through the cache
   unsigned hot_max_taken;  // particles a work-group takes from a problem before it flushes the cache (< 2^24)
@@ -167,64 +168,101 @@ struct LaunchParams {
 };
 
 // Per-particle Philox stream: counter = {c, 0, pid_lo, pid_hi}, key = (k0, k1).
+// Every draw site takes whole blocks of four 32-bit words, so all lanes of a
+// wavefront run the generator at the same sites, once per site (a per-lane
+// leftover would desynchronise the lanes, and the generator would then run at
+// every draw for some of them).  A uniform takes 53 bits (two words) where its
+// tail matters -- the exponential and geometric free-flight draws and the
+// Box-Muller radius -- and 32 bits (one word) where it is compared with a
+// probability or turned into an angle: azimuths, the polar angle's table
+// lookup, mode conversion, boundary decisions, birth directions.  2^-32 is far
+// below the 1e-8 cdf steps of the reference's angle tables.
 struct Rng {
   uint32_t k0, k1, plo, phi;
   uint32_t c;
-  bool has;
-  double spare;
-  U4 w32;
-  int n32;
   __device__ __forceinline__ void start(uint64_t pid) {
     plo = (uint32_t)pid;
     phi = (uint32_t)(pid >> 32);
     c = 0u;
-    has = false;
-    n32 = 0;
   }
-  __device__ __forceinline__ double next() {
-    if (has) {
-      has = false;
-      return spare;
-    }
+  __device__ __forceinline__ U4 block() {
     U4 in;
     in.x = c;
     in.y = 0u;
     in.z = plo;
     in.w = phi;
-    U4 r = philox10(in, k0, k1);
     c += 1u;
-    spare = u53(r.z, r.w);
-    has = true;
-    return u53(r.x, r.y);
+    return philox10(in, k0, k1);
   }
-  // uniform in (0, 1] with 32 random bits (FP32 flight): one Philox word each
-  __device__ __forceinline__ double next32() {
-    if (n32 == 0) {
-      U4 in;
-      in.x = c;
-      in.y = 0u;
-      in.z = plo;
-      in.w = phi;
-      w32 = philox10(in, k0, k1);
-      c += 1u;
-      n32 = 4;
-    }
-    uint32_t w = (n32 == 4) ? w32.x : (n32 == 3) ? w32.y : (n32 == 2) ? w32.z : w32.w;
-    n32 -= 1;
-    return ((double)w + 1.0) * (1.0 / 4294967296.0);
-  }
+  // uniform in (0, 1] from one word
+  __device__ __forceinline__ static double w32(uint32_t w) { return ((double)w + 1.0) * (1.0 / 4294967296.0); }
+  // a free-flight uniform of the flight precision R: 53 bits from words (a, b), or 32 from a
   template <typename R>
-  __device__ __forceinline__ double u() {
-    if (sizeof(R) == 8) return next();
-    return next32();
+  __device__ __forceinline__ static double wflight(uint32_t a, uint32_t b) {
+    return sizeof(R) == 8 ? u53(a, b) : w32(a);
+  }
+  // the uniforms of one scattering: azimuth, polar angle and conversion (32 bits)
+  // and the next free flight (53 bits with FP64 flight): one block, or two for
+  // an elastic medium with FP64 flight
+  template <typename R, bool AC>
+  __device__ __forceinline__ void scatter4(double& uphi, double& uth, double& uconv, double& ufl) {
+    const U4 r = block();
+    uphi = w32(r.x);
+    uth = w32(r.y);
+    if (sizeof(R) != 8) {
+      uconv = w32(r.z);
+      ufl = w32(r.w);
+    } else if (AC) {
+      uconv = 0.0;
+      ufl = u53(r.z, r.w);
+    } else {
+      uconv = w32(r.z);
+      const U4 r2 = block();
+      ufl = u53(r2.x, r2.y);
+    }
+  }
+  // two free-flight uniforms of the flight precision R (one block)
+  template <typename R>
+  __device__ __forceinline__ void flight2(double& a, double& b) {
+    const U4 r = block();
+    a = wflight<R>(r.x, r.y);
+    b = sizeof(R) == 8 ? u53(r.z, r.w) : w32(r.y);
+  }
+  // two 32-bit uniforms (one block)
+  __device__ __forceinline__ void pair32(double& a, double& b) {
+    const U4 r = block();
+    a = w32(r.x);
+    b = w32(r.y);
+  }
+};
+
+// Words of the generator taken one at a time (a birth's draws; every lane of a
+// birth follows the same sequence, so the blocks stay aligned across lanes).
+struct WordStream {
+  Rng& r;
+  U4 b;
+  int n;
+  __device__ __forceinline__ explicit WordStream(Rng& rng) : r(rng), n(0) {}
+  __device__ __forceinline__ uint32_t word() {
+    if (n == 0) {
+      b = r.block();
+      n = 4;
+    }
+    const uint32_t w = n == 4 ? b.x : (n == 3 ? b.y : (n == 2 ? b.z : b.w));
+    n -= 1;
+    return w;
+  }
+  __device__ __forceinline__ double u32() { return Rng::w32(word()); }
+  __device__ __forceinline__ double u53() {
+    const uint32_t a = word();
+    return rtmc::u53(a, word());
   }
 };
 
 // index of the next sub-step with a success, trials of probability 1-exp(-lam)
-// (ilam = 1 / lam)
+// (ilam = 1 / lam), from the uniform u
 template <typename R>
-__device__ __forceinline__ int64_t geom_after(Rng& r, double lam, double ilam) {
-  double u = r.u<R>();
+__device__ __forceinline__ int64_t geom_from(double u, double lam, double ilam) {
   if (!(lam > 0.0)) return kInf;
   double e = (sizeof(R) == 8) ? -log(u) : (double)(-logf((float)u));
   double g = ceil(e * ilam);
@@ -336,17 +374,18 @@ __device__ __forceinline__ void bin_setup(const LaunchParams& p, int dim, const 
 // FP64 value).
 template <int DIM>
 __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x[3], double d[3], double q[3]) {
-  // The same draws and the same angles as initializeParticle.m:17-147; each
+  // The same variates and the same angles as initializeParticle.m:17-147; each
   // angle 2 pi u enters through sincospi(2u), and a polar angle acos(c) through
-  // its cosine c and sine sqrt(1 - c^2) (no acos, no sincos).
+  // its cosine c and sine sqrt(1 - c^2).
+  WordStream ws(rng);
   if (S.type == RTMC_SRC_POINT) {
     double r;
     if (S.radial.n == 0) {
-      double u1 = rng.next(), u2 = rng.next();
+      const double u1 = ws.u53(), u2 = ws.u32();
       double g = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
       r = fabs(g * S.lambda);  // initializeParticle.m:23
     } else {
-      r = table_eval(S.radial.x, S.radial.y, S.radial.n, S.radial.extrap, rng.next());  // :25-27
+      r = table_eval(S.radial.x, S.radial.y, S.radial.n, S.radial.extrap, ws.u53());  // :25-27
     }
     // polar angles as (cos, sin); azimuths as turns
     double cp0, sp0, cp, sp;
@@ -354,20 +393,20 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
       cp0 = cp = cos(kPi / 2);  // 6.12e-17, as MATLAB's cos(pi/2)
       sp0 = sp = 1.0;
     } else {
-      cp0 = -1.0 + 2.0 * rng.next();
+      cp0 = -1.0 + 2.0 * ws.u32();
       sp0 = sqrt(fmax(0.0, 1.0 - cp0 * cp0));
-      cp = -1.0 + 2.0 * rng.next();
+      cp = -1.0 + 2.0 * ws.u32();
       sp = sqrt(fmax(0.0, 1.0 - cp * cp));
     }
-    double t0 = rng.next();
-    double t = rng.next();
+    const double t0 = ws.u32();
+    double t = ws.u32();
     if (S.direction == RTMC_DIR_OUTGOING) {
       t = t0;
       cp = cp0;
       sp = sp0;
     } else if (S.direction == RTMC_DIR_UPPER) {
-      t = rng.next();
-      cp = rng.next();
+      t = ws.u32();
+      cp = ws.u32();
       sp = sqrt(fmax(0.0, 1.0 - cp * cp));
     }
     double st, ct;
@@ -382,7 +421,7 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
       double c[3];
       cross3(d, q, c);
       double sa, ca;
-      sincospi(2.0 * rng.next(), &sa, &ca);
+      sincospi(2.0 * ws.u32(), &sa, &ca);
       for (int k = 0; k < 3; ++k) q[k] = q[k] * ca + c[k] * sa;
     }
     double st0, ct0;
@@ -391,26 +430,27 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
     x[1] = S.position[1] + r * (st0 * sp0);
     x[2] = S.position[2] + r * cp0;
   } else {
-    double u1 = rng.next(), u2 = rng.next();
+    const double u1 = ws.u53(), u2 = ws.u32();
     double g = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
     double c_long = g * S.lambda / 2.0;
     double c1, c2;
+    const double a = ws.u32(), b = ws.u32();
     if (S.aperture == RTMC_APERTURE_RECTANGLE) {
-      c1 = (rng.next() - 0.5) * S.extent[0];
-      c2 = (rng.next() - 0.5) * S.extent[1];
+      c1 = (a - 0.5) * S.extent[0];
+      c2 = (b - 0.5) * S.extent[1];
     } else {
-      double rho = S.extent[0] * sqrt(rng.next());
+      double rho = S.extent[0] * sqrt(a);
       double sa, ca;
-      sincospi(2.0 * rng.next(), &sa, &ca);
+      sincospi(2.0 * b, &sa, &ca);
       c1 = rho * ca;
       c2 = rho * sa;
     }
     for (int k = 0; k < 3; ++k)
       x[k] = ((S.position[k] + c_long * S.normal[k]) + c1 * S.basis_u[k]) + c2 * S.basis_v[k];
     for (int k = 0; k < 3; ++k) d[k] = S.normal[k];
-    double s, c;
-    sincospi(2.0 * rng.next(), &s, &c);
-    for (int k = 0; k < 3; ++k) q[k] = c * S.basis_u[k] + s * S.basis_v[k];
+    double sn, cs;
+    sincospi(2.0 * ws.u32(), &sn, &cs);
+    for (int k = 0; k < 3; ++k) q[k] = cs * S.basis_u[k] + sn * S.basis_v[k];
   }
 }
 
@@ -664,7 +704,6 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     rng.k0 = P.k0;
     rng.k1 = P.k1;
     rng.start(0);
-    rng.spare = 0.0;
     R xb[3] = {R(0), R(0), R(0)}, d[3] = {R(0), R(0), R(1)}, q[3] = {R(1), R(0), R(0)};
     int isP = 1;
     R h = R(0);                     // sub-stepped: step length v dt_s; event-driven: speed v
@@ -721,7 +760,11 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
       if (pm == 0ull && fm == 0ull) break;
       if (__popcll(pm) >= p.batch || fm == 0ull) {
         // ---------------- EVENT step ----------------
-        const uint64_t nm = __ballot(need);
+        // Births wait until birth_min lanes want a particle, unless no lane can fly or
+        // no event is pending: the birth path then runs for several lanes at once
+        // instead of for the one or two that ran out in each event step.
+        uint64_t nm = __ballot(need);
+        if (nm != 0ull && __popcll(nm) < p.birth_min && fm != 0ull && (pm & ~nm) != 0ull) nm = 0ull;
         if (nm) {
           // hand out particle indices to the lanes that need one
           const int n = __popcll(nm);
@@ -746,7 +789,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               nb1 = min((int64_t)base + (int64_t)p.chunk, P.n);
             }
           }
-          if (need) {
+          if (need) {  // nm != 0: this step births
             const int r = __popcll(nm & lt_mask);
             int64_t id = -1;
             if (r < L) id = pool_next + r;
@@ -755,6 +798,8 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               rng.start(P.pid0 + (uint64_t)id);
               if (RUNS3) psub = (int)(id / p.run_n);
               birth<DIM>(P.S, rng, xb, d, q);
+              double gs, gk;
+              rng.flight2<R>(gs, gk);
               dbin = dir_bin(p, bs, sedges, DIM, d[2]);
               isP = (!AC && P.S.polarization == 1) ? 0 : 1;
               it = 1;
@@ -765,13 +810,13 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               if (QUANT) {
                 h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
                 kb = 0;
-                Ks = geom_after<R>(rng, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
-                Kk = P.useQ ? geom_after<R>(rng, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]) : kInf;
+                Ks = geom_from<R>(gs, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
+                Kk = P.useQ ? geom_from<R>(gk, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]) : kInf;
                 next_event();
               } else {
                 h = (R)(isP ? M.vp : M.vs);
                 tb = 0.0;
-                ts = -log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
+                ts = -log(gs) * (isP ? M.mftP : M.mftS);
               }
             } else if (exhausted || r - L >= nb1 - nb0) {
               need = false;  // nothing left to trace
@@ -813,10 +858,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                   break;
                 }
                 double uA = 0.0, uB = 0.0;
-                if (!AC) {
-                  uA = rng.u<R>();
-                  uB = rng.u<R>();
-                }
+                if (!AC) rng.pair32(uA, uB);
                 if (B.dir < 4) plane_reflect(M, AC, B, x, d, q, isP, uA, uB);
                 else cyl_reflect(M, AC, B, x, d, q, isP, uA, uB);
               }
@@ -825,14 +867,16 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                   // mode changed at the boundary: this sub-step's scattering test and
                   // the removal tests from the next one use the new mode
                   h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
-                  Ks = j - 1 + geom_after<R>(rng, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
-                  Kk = P.useQ ? j + geom_after<R>(rng, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]) : kInf;
+                  double gs, gk;
+                  rng.flight2<R>(gs, gk);
+                  Ks = j - 1 + geom_from<R>(gs, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
+                  Kk = P.useQ ? j + geom_from<R>(gk, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]) : kInf;
                 }
                 if (Ks == j) {
                   const int before = isP;
-                  double uphi = rng.u<R>();
-                  double uth = rng.u<R>();
-                  double uconv = AC ? 0.0 : rng.u<R>();
+                  // one draw site: azimuth, polar angle, conversion, next scattering
+                  double uphi, uth, uconv, ugs;
+                  rng.scatter4<R, AC>(uphi, uth, uconv, ugs);
                   double th = scatter_angles_t<DIM, R, decltype(lookup), true>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
                   if (ADJ) {
                     nsc += 1;
@@ -842,9 +886,13 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                   nscat += 1;
                   if (!AC && isP != before) {
                     h = (R)((isP ? M.vp : M.vs) * P.dt_sub);
-                    if (P.useQ) Kk = j + geom_after<R>(rng, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]);
+                    if (P.useQ) {
+                      double gk, unused;
+                      rng.flight2<R>(gk, unused);
+                      Kk = j + geom_from<R>(gk, P.lamK[isP ? 0 : 1], P.ilamK[isP ? 0 : 1]);
+                    }
                   }
-                  Ks = j + geom_after<R>(rng, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
+                  Ks = j + geom_from<R>(ugs, P.lamS[isP ? 0 : 1], P.ilamS[isP ? 0 : 1]);
                 }
                 xb[0] = x[0];
                 xb[1] = x[1];
@@ -868,9 +916,8 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             const R f = (R)((double)h * (ts - tb));
             for (int k = 0; k < 3; ++k) xb[k] = xb[k] + f * d[k];
             tb = ts;
-            double uphi = rng.u<R>();
-            double uth = rng.u<R>();
-            double uconv = AC ? 0.0 : rng.u<R>();
+            double uphi, uth, uconv, uexp;
+            rng.scatter4<R, AC>(uphi, uth, uconv, uexp);
             double th = scatter_angles_t<DIM, R, decltype(lookup), true>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
             if (ADJ) {
               nsc += 1;
@@ -880,7 +927,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             dbin = dir_bin(p, bs, sedges, DIM, d[2]);
             nscat += 1;
             h = (R)(isP ? M.vp : M.vs);
-            ts = tb - log(rng.u<R>()) * (isP ? M.mftP : M.mftS);
+            ts = tb - log(uexp) * (isP ? M.mftP : M.mftS);
           }
         }
       } else {
@@ -922,7 +969,9 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                                               (uint32_t)it);
             bool cached = false;
             if (j >= 0 && p.hot_slots > 0 && it < p.hot_times) {
-              const int s = (int)(((unsigned long long)j * 0x9E3779B97F4A7C15ull) >> p.hot_shift);
+              // multiplicative hash of the (< 2^40) T index folded to 32 bits
+              const uint32_t hj = (uint32_t)((unsigned long long)j ^ ((unsigned long long)j >> 27));
+              const int s = (int)((hj * 0x9E3779B1u) >> p.hot_shift);
               const unsigned long long mine = (unsigned long long)j << 24;
               const unsigned long long cur = hslot[s];
               if (cur != kEmpty && (cur >> 24) == (unsigned long long)j) {
