@@ -215,11 +215,21 @@ struct MatDev {
 // during the current sub-step (propagateParticleSmallDt.m:167-581).
 // uA, uB: the two boundary variates (P: reflect-or-convert; S: SV/SH split;
 // SV: reflect-or-convert).
+// The boundary code indexes vectors only with constants and, where branches
+// write different components, works on local copies written back once at the
+// end: a run-time index, or stores to different addresses merged at a join,
+// would keep the particle's vectors in the private segment.
+template <typename T>
+__device__ __forceinline__ T comp3(const T a[3], int b) {
+  const T a0 = a[0], a1 = a[1], a2 = a[2];
+  return b == 0 ? a0 : (b == 1 ? a1 : a2);
+}
+
 template <typename T>
 __device__ __forceinline__ bool plane_crossed(const rtmc_boundary& B, const T x[3], const T x1[3]) {
   int b = B.dir - 1;
   const T v = (T)B.val;
-  return (x[b] - v) * (x1[b] - v) < T(0);  // :171-172
+  return (comp3(x, b) - v) * (comp3(x1, b) - v) < T(0);  // :171-172
 }
 
 template <typename T>
@@ -230,8 +240,8 @@ __device__ __forceinline__ bool cyl_outside(const rtmc_boundary& B, const T x[3]
 
 // MaterialClass.MaterialInterface branch at plane of normal e_b
 // (MaterialClass.m:1587-1735): 0 P->P, 1 P->SV, 2 SH, 3 SV->SV, 4 SV->P.
-template <typename T>
-__device__ __forceinline__ int interface_plane(const MatDev& M, int b, const T d[3], const T q[3], int isP, double uA,
+template <int b, typename T>
+__device__ __forceinline__ int interface_plane(const MatDev& M, const T d[3], const T q[3], int isP, double uA,
                                                double uB) {
   T nv[3] = {T(0), T(0), T(0)};
   nv[b] = T(1);
@@ -269,60 +279,78 @@ __device__ __forceinline__ int interface_plane(const MatDev& M, int b, const T d
   return (uB <= R) ? 3 : 4;                                              // :1702-1703
 }
 
-// Applies the reflection of a particle that crossed plane boundary B
-// (acoustic :184-187; elastic :190-321).
-template <typename T>
-__device__ __forceinline__ void plane_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, T x[3], T d[3],
-                                              T q[3], int& isP, double uA, double uB) {
-  int b = B.dir - 1;
-  const T v2 = (T)(2 * B.val);
+// Reflection at the plane x_b = val (acoustic :184-187; elastic :190-321).
+template <int b, typename T>
+__device__ __forceinline__ void plane_reflect_axis(const MatDev& M, bool acoustic, double val, T xo[3], T dout[3],
+                                                   T qo[3], int& isPo, double uA, double uB) {
+  T x[3] = {xo[0], xo[1], xo[2]}, d[3] = {dout[0], dout[1], dout[2]}, q[3] = {qo[0], qo[1], qo[2]};
+  int isP = isPo;
+  const T v2 = (T)(2 * val);
   if (acoustic) {
     x[b] = v2 - x[b];
     d[b] = -d[b];
-    return;
-  }
-  int br = interface_plane(M, b, d, q, isP, uA, uB);
-  x[b] = v2 - x[b];  // :198
-  if (br == 0 || br == 2 || br == 3) {
-    d[b] = -d[b];
-    q[b] = -q[b];
-  } else if (br == 1) {  // P -> SV :212-248
-    isP = 0;
-    T ratio = (T)(M.vs / M.vp);
-    T dn = d[b];
-    T t[3] = {d[0], d[1], d[2]};
-    t[b] = T(0);
-    for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
-    T sin2 = sumsq3(t);
-    T cv = sqrt(fmax(T(0), T(1) - sin2));
-    t[b] = -sgn(dn) * cv;
-    T nn = sqrt(sumsq3(t));
-    for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
-    T nv[3] = {T(0), T(0), T(0)};
-    nv[b] = T(1);
-    T ax[3];
-    cross3(d, nv, ax);
-    T na = sqrt(sumsq3(ax));
-    for (int k = 0; k < 3; ++k) ax[k] = ax[k] / na;
-    cross3(ax, d, q);
-  } else {  // SV -> P :274-320
-    isP = 1;
-    T ratio = (T)(M.vp / M.vs);
-    T dn = d[b];
-    T t[3] = {d[0], d[1], d[2]};
-    t[b] = T(0);
-    for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
-    T sin2 = sumsq3(t);
-    if (sin2 <= T(1)) {
-      T cv = sqrt(T(1) - sin2);
+  } else {
+    int br = interface_plane<b>(M, d, q, isP, uA, uB);
+    x[b] = v2 - x[b];  // :198
+    if (br == 0 || br == 2 || br == 3) {
+      d[b] = -d[b];
+      q[b] = -q[b];
+    } else if (br == 1) {  // P -> SV :212-248
+      isP = 0;
+      T ratio = (T)(M.vs / M.vp);
+      T dn = d[b];
+      T t[3] = {d[0], d[1], d[2]};
+      t[b] = T(0);
+      for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
+      T sin2 = sumsq3(t);
+      T cv = sqrt(fmax(T(0), T(1) - sin2));
       t[b] = -sgn(dn) * cv;
       T nn = sqrt(sumsq3(t));
       for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
-    } else {
-      isP = 0;  // total reflection fallback :313-319
-      d[b] = -d[b];
+      T nv[3] = {T(0), T(0), T(0)};
+      nv[b] = T(1);
+      T ax[3];
+      cross3(d, nv, ax);
+      T na = sqrt(sumsq3(ax));
+      for (int k = 0; k < 3; ++k) ax[k] = ax[k] / na;
+      cross3(ax, d, q);
+    } else {  // SV -> P :274-320
+      isP = 1;
+      T ratio = (T)(M.vp / M.vs);
+      T dn = d[b];
+      T t[3] = {d[0], d[1], d[2]};
+      t[b] = T(0);
+      for (int k = 0; k < 3; ++k) t[k] = ratio * t[k];
+      T sin2 = sumsq3(t);
+      if (sin2 <= T(1)) {
+        T cv = sqrt(T(1) - sin2);
+        t[b] = -sgn(dn) * cv;
+        T nn = sqrt(sumsq3(t));
+        for (int k = 0; k < 3; ++k) d[k] = t[k] / nn;
+      } else {
+        isP = 0;  // total reflection fallback :313-319
+        d[b] = -d[b];
+      }
     }
   }
+  for (int k = 0; k < 3; ++k) {
+    xo[k] = x[k];
+    dout[k] = d[k];
+    qo[k] = q[k];
+  }
+  isPo = isP;
+}
+
+// Applies the reflection of a particle that crossed plane boundary B.
+template <typename T>
+__device__ __forceinline__ void plane_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, T x[3], T d[3],
+                                              T q[3], int& isP, double uA, double uB) {
+  if (B.dir == 1)
+    plane_reflect_axis<0>(M, acoustic, B.val, x, d, q, isP, uA, uB);
+  else if (B.dir == 2)
+    plane_reflect_axis<1>(M, acoustic, B.val, x, d, q, isP, uA, uB);
+  else
+    plane_reflect_axis<2>(M, acoustic, B.val, x, d, q, isP, uA, uB);
 }
 
 template <typename T>
@@ -338,8 +366,10 @@ __device__ __forceinline__ void reflect_dq(T d[3], T q[3], const T nv[3]) {
 // Applies the reflection of a particle outside cylinder B
 // (propagateParticleSmallDt.m:344-578).
 template <typename T>
-__device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, T x[3], T d[3],
-                                            T q[3], int& isP, double uA, double uB) {
+__device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, const rtmc_boundary& B, T xo[3],
+                                            T dout[3], T qo[3], int& isPo, double uA, double uB) {
+  T x[3] = {xo[0], xo[1], xo[2]}, d[3] = {dout[0], dout[1], dout[2]}, q[3] = {qo[0], qo[1], qo[2]};
+  int isP = isPo;
   T r = sqrt(x[0] * x[0] + x[1] * x[1]);
   T ov = r - (T)B.val;
   T nv[3] = {x[0] / r, x[1] / r, T(0)};
@@ -347,9 +377,7 @@ __device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, cons
   if (acoustic) {                                               // :354-359
     T dp = dot3(d, nv);
     for (int k = 0; k < 3; ++k) d[k] = d[k] - T(2) * dp * nv[k];
-    return;
-  }
-  if (isP) {  // :373-455
+  } else if (isP) {  // :373-455
     T ct = dot3(d, nv);
     double thd = acosd(fmin(T(1), fmax(T(-1), ct)));
     double R = table_eval(M.zdeg, M.zRpp, M.nz, RTMC_EXTRAP_NEAREST, thd);
@@ -422,6 +450,12 @@ __device__ __forceinline__ void cyl_reflect(const MatDev& M, bool acoustic, cons
       }
     }
   }
+  for (int k = 0; k < 3; ++k) {
+    xo[k] = x[k];
+    dout[k] = d[k];
+    qo[k] = q[k];
+  }
+  isPo = isP;
 }
 
 // ------------------------------------------------------------------

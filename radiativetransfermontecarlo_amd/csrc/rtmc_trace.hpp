@@ -281,8 +281,8 @@ __device__ __forceinline__ int64_t predict_boundary(const rtmc_boundary& B, cons
   const double h = h_;
   if (B.dir < 4) {
     int b = B.dir - 1;
-    double s0 = xb[b] - B.val;
-    double r = h * d[b];
+    double s0 = comp3(xb, b) - B.val;
+    double r = h * comp3(d, b);
     if (!(s0 * r < 0.0)) return kInf;  // moving away, parallel, or on the plane
     double ns = fabs(s0) / fabs(r);
     if (!(ns < 1e15)) return kInf;
@@ -377,7 +377,12 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
   // The same variates and the same angles as initializeParticle.m:17-147; each
   // angle 2 pi u enters through sincospi(2u), and a polar angle acos(c) through
   // its cosine c and sine sqrt(1 - c^2).
+  // Both branches fill the same locals and the arrays are written once at the
+  // end: stores to different arrays at the ends of the two branches would be
+  // merged into one store through a pointer select, which keeps the arrays in
+  // the private segment.
   WordStream ws(rng);
+  double x0, x1, x2, d0, d1, d2, q0, q1, q2;
   if (S.type == RTMC_SRC_POINT) {
     double r;
     if (S.radial.n == 0) {
@@ -411,24 +416,27 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
     }
     double st, ct;
     sincospi(2.0 * t, &st, &ct);
-    d[0] = ct * sp;
-    d[1] = st * sp;
-    d[2] = cp;
-    q[0] = -st;
-    q[1] = ct;
-    q[2] = 0.0;
+    d0 = ct * sp;
+    d1 = st * sp;
+    d2 = cp;
+    q0 = -st;
+    q1 = ct;
+    q2 = 0.0;
     if (DIM == 3) {
+      const double dv[3] = {d0, d1, d2}, qv[3] = {q0, q1, q2};
       double c[3];
-      cross3(d, q, c);
+      cross3(dv, qv, c);
       double sa, ca;
       sincospi(2.0 * ws.u32(), &sa, &ca);
-      for (int k = 0; k < 3; ++k) q[k] = q[k] * ca + c[k] * sa;
+      q0 = q0 * ca + c[0] * sa;
+      q1 = q1 * ca + c[1] * sa;
+      q2 = q2 * ca + c[2] * sa;
     }
     double st0, ct0;
     sincospi(2.0 * t0, &st0, &ct0);
-    x[0] = S.position[0] + r * (ct0 * sp0);
-    x[1] = S.position[1] + r * (st0 * sp0);
-    x[2] = S.position[2] + r * cp0;
+    x0 = S.position[0] + r * (ct0 * sp0);
+    x1 = S.position[1] + r * (st0 * sp0);
+    x2 = S.position[2] + r * cp0;
   } else {
     const double u1 = ws.u53(), u2 = ws.u32();
     double g = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
@@ -445,13 +453,27 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
       c1 = rho * ca;
       c2 = rho * sa;
     }
-    for (int k = 0; k < 3; ++k)
-      x[k] = ((S.position[k] + c_long * S.normal[k]) + c1 * S.basis_u[k]) + c2 * S.basis_v[k];
-    for (int k = 0; k < 3; ++k) d[k] = S.normal[k];
+    x0 = ((S.position[0] + c_long * S.normal[0]) + c1 * S.basis_u[0]) + c2 * S.basis_v[0];
+    x1 = ((S.position[1] + c_long * S.normal[1]) + c1 * S.basis_u[1]) + c2 * S.basis_v[1];
+    x2 = ((S.position[2] + c_long * S.normal[2]) + c1 * S.basis_u[2]) + c2 * S.basis_v[2];
+    d0 = S.normal[0];
+    d1 = S.normal[1];
+    d2 = S.normal[2];
     double sn, cs;
     sincospi(2.0 * ws.u32(), &sn, &cs);
-    for (int k = 0; k < 3; ++k) q[k] = cs * S.basis_u[k] + sn * S.basis_v[k];
+    q0 = cs * S.basis_u[0] + sn * S.basis_v[0];
+    q1 = cs * S.basis_u[1] + sn * S.basis_v[1];
+    q2 = cs * S.basis_u[2] + sn * S.basis_v[2];
   }
+  x[0] = x0;
+  x[1] = x1;
+  x[2] = x2;
+  d[0] = d0;
+  d[1] = d1;
+  d[2] = d2;
+  q[0] = q0;
+  q[1] = q1;
+  q[2] = q2;
 }
 
 template <int DIM, typename R>
