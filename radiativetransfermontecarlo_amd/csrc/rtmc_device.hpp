@@ -77,6 +77,15 @@ struct SlotDraws {
   }
 };
 
+// Global-memory pointers: loads and atomics through them compile to global, not
+// flat, instructions (a flat access counts against the LDS wait counter too, so
+// it holds up the LDS and cross-lane operations that wait on that counter).
+#define RTMC_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ RTMC_GLOBAL T* gl(T* p) {
+  return (RTMC_GLOBAL T*)p;
+}
+
 // ------------------------------------------------------------------
 // Small vector helpers, generic in the real type (double: the reference's
 // arithmetic; float: the FP32 free-flight variant).  Sums are left to right,
@@ -107,8 +116,8 @@ __device__ __forceinline__ void sc2pi(float u, float* s, float* c) { sincospif(2
 // griddedInterpolant(x, y)(q): linear inside, linear or nearest outside
 // (MaterialClass.m:2462, initializeParticle.m:201, interp1(...,'extrap')
 // MaterialClass.m:1674).  Interval = largest k <= n-2 with x[k] <= q.
-__device__ __forceinline__ double table_eval(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
-                                             int extrap, double q) {
+template <typename PD>
+__device__ __forceinline__ double table_eval(PD __restrict__ x, PD __restrict__ y, int64_t n, int extrap, double q) {
   if (q != q) return q;
   int64_t k;
   if (q < x[0]) {
@@ -132,8 +141,9 @@ __device__ __forceinline__ double table_eval(const double* __restrict__ x, const
 
 // Same lookup, restricted to the knot range [lo, hi] known to bracket q
 // (guide-table search: lo = largest knot <= q is inside [lo, hi]).
-__device__ __forceinline__ double table_eval_bracket(const double* __restrict__ x, const double* __restrict__ y,
-                                                     int64_t n, int extrap, double q, int64_t lo, int64_t hi) {
+template <typename PD>
+__device__ __forceinline__ double table_eval_bracket(PD __restrict__ x, PD __restrict__ y, int64_t n, int extrap,
+                                                     double q, int64_t lo, int64_t hi) {
   if (q != q) return q;
   int64_t k;
   if (q < x[0]) {
@@ -163,8 +173,8 @@ __device__ __forceinline__ double table_eval_bracket(const double* __restrict__ 
 // table_eval with the interval bracketed exactly, x[lo] <= q < x[hi] (a guide
 // built from this non-decreasing x): no widening checks, so the knots are read
 // once, after the few search steps (the bracket usually holds one interval).
-__device__ __forceinline__ double table_eval_exact_bracket(const double* __restrict__ x,
-                                                           const double* __restrict__ y, int64_t n, int extrap,
+template <typename PD>
+__device__ __forceinline__ double table_eval_exact_bracket(PD __restrict__ x, PD __restrict__ y, int64_t n, int extrap,
                                                            double q, int64_t lo, int64_t hi) {
   if (q != q) return q;
   int64_t k;

@@ -390,7 +390,7 @@ __device__ __forceinline__ void birth64(const rtmc_source& S, Rng& rng, double x
       double g = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
       r = fabs(g * S.lambda);  // initializeParticle.m:23
     } else {
-      r = table_eval(S.radial.x, S.radial.y, S.radial.n, S.radial.extrap, ws.u53());  // :25-27
+      r = table_eval(gl(S.radial.x), gl(S.radial.y), S.radial.n, S.radial.extrap, ws.u53());  // :25-27
     }
     // polar angles as (cos, sin); azimuths as turns
     double cp0, sp0, cp, sp;
@@ -499,8 +499,10 @@ __device__ __forceinline__ int dir_bin(const LaunchParams& p, const BinSetup& bs
   if (cp > R(1)) cp = R(1);
   if (cp < R(-1)) cp = R(-1);
   const R psi = (dim == 3) ? cp : acos(cp);
-  const double* e = p.edges_total > 0 ? sedges + p.eoff[3] : p.edges[3];
-  return bin_guess2(e, p.ne[3], bs.e0[3], bs.ehi[3], bs.invw[3], (double)psi);
+  // two calls rather than one through a selected pointer: each reads its own
+  // memory (LDS or global) instead of through flat addressing
+  if (p.edges_total > 0) return bin_guess2(sedges + p.eoff[3], p.ne[3], bs.e0[3], bs.ehi[3], bs.invw[3], (double)psi);
+  return bin_guess2(p.edges[3], p.ne[3], bs.e0[3], bs.ehi[3], bs.invw[3], (double)psi);
 }
 
 // Index of the observation's bin over the binned variables (observeTime.m:15-45
@@ -647,6 +649,8 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     const int cur = __builtin_amdgcn_readfirstlane(s_cur);
     if (cur < 0) break;
     const ProbDev& P = probs[cur];
+    RTMC_GLOBAL uint32_t* const PE = gl(P.E);
+    RTMC_GLOBAL uint32_t* const PT = gl(P.T);
     const MatDev& M = P.M;
     unsigned long long* const work = p.work + cur;
 
@@ -690,6 +694,8 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     // table_eval), bracketed by the LDS guide when present
     auto lookup = [&](int t, double u) -> double {
       const TableDev& T = M.inv[t];
+      const RTMC_GLOBAL double* const Tx = gl(T.x);
+      const RTMC_GLOBAL double* const Ty = gl(T.y);
       if (p.gg_log2 > 0) {
         // cell of u in [0, 1] (u * G is exact for G a power of two); one 8-byte read
         // brackets the knot, usually to one interval
@@ -697,19 +703,19 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
         if (u >= 0.0 && u <= 1.0) {
           int c = (int)(u * (double)G);
           c = c > G - 1 ? G - 1 : c;
-          const int* g = P.gg + (size_t)t * (size_t)(G + 1) + c;
-          return table_eval_exact_bracket(T.x, T.y, T.n, T.extrap, u, g[0], (int64_t)g[1] + 1);
+          const RTMC_GLOBAL int* g = gl(P.gg) + (size_t)t * (size_t)(G + 1) + c;
+          return table_eval_exact_bracket(Tx, Ty, T.n, T.extrap, u, g[0], (int64_t)g[1] + 1);
         }
-        return table_eval(T.x, T.y, T.n, T.extrap, u);
+        return table_eval(Tx, Ty, T.n, T.extrap, u);
       }
       if (p.guide_n > 0 && u >= s_tx0[t] && u <= s_tx1[t]) {
         const int G = p.guide_n;
         int c = (int)(u * (double)G);
         c = c < 0 ? 0 : (c > G - 1 ? G - 1 : c);
         const int* g = sguide + t * (G + 1);
-        return table_eval_bracket(T.x, T.y, T.n, T.extrap, u, g[c], (int64_t)g[c + 1] + 1);
+        return table_eval_bracket(Tx, Ty, T.n, T.extrap, u, g[c], (int64_t)g[c + 1] + 1);
       }
-      return table_eval(T.x, T.y, T.n, T.extrap, u);
+      return table_eval(Tx, Ty, T.n, T.extrap, u);
     };
     uint32_t nscat = 0, ndep = 0, nbnd = 0, nkill = 0;  // this lane's, for this problem
     // adjoint sums of this lane: w A, w (sum over events of dlogsigma - dlogSigma), w k
@@ -742,12 +748,12 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     auto flush_run = [&]() {
       const unsigned tr = (unsigned)trun;
       const unsigned w = tr / 3u, s = tr - 3u * w;
-      unsigned long long* a = (unsigned long long*)P.T + (size_t)(uint32_t)rsub * (size_t)p.run_words +
+      RTMC_GLOBAL unsigned long long* a = (RTMC_GLOBAL unsigned long long*)PT + (size_t)(uint32_t)rsub * (size_t)p.run_words +
                               ((size_t)(uint32_t)jrow * 3u + s) * (size_t)(uint32_t)p.tw + w;
       if (lrun == 1) {
         // the first counter (bits 0-20) alone: a 32-bit atomic on the low half, which
         // the memory side executes faster than a 64-bit one (no carry leaves bit 20)
-        __hip_atomic_fetch_add((uint32_t*)a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add((RTMC_GLOBAL uint32_t*)a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
         __hip_atomic_fetch_add(a, lrun == 2 ? 0x200001ull : 0x40000200001ull, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -928,7 +934,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               has = false;
               need = true;
               if (TMAJ && tpend >= 0) {
-                __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(PT + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 tpend = -1;
               }
               if (RUNS3 && jrow >= 0) flush_run();
@@ -981,7 +987,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             const bool tfit = !p.tcomp || dbin == bs.tdir;
             const uint32_t tbase = p.tcomp ? (uint32_t)bs.tdir * p.stride[3] : 0u;
             if (sp >= 0 && !tfit) {
-              __hip_atomic_fetch_add(P.E + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_fetch_add(PE + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               ndep += 1;
             }
             const int64_t j = (sp < 0 || !tfit)
@@ -1027,21 +1033,22 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                 }
               }
             } else if (cached) {
-              if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (tpend >= 0) __hip_atomic_fetch_add(PT + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
             } else if (tpend >= 0 && j == tpend + 1) {
               // the same counter row at two consecutive times: one 64-bit atomic on the
               // aligned pair, in T from an even time, in T2 (shifted by one) from an odd one
-              unsigned long long* w = (tpend & 1) ? (unsigned long long*)P.T2 + ((tpend - 1) >> 1)
-                                                  : (unsigned long long*)P.T + (tpend >> 1);
+              RTMC_GLOBAL unsigned long long* w =
+                  (tpend & 1) ? (RTMC_GLOBAL unsigned long long*)gl(P.T2) + ((tpend - 1) >> 1)
+                              : (RTMC_GLOBAL unsigned long long*)PT + (tpend >> 1);
               __hip_atomic_fetch_add(w, 0x100000001ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
             } else {
-              if (tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (tpend >= 0) __hip_atomic_fetch_add(PT + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               tpend = -1;
               if (j >= 0) {
                 if (it + 1 < p.n_times && (!(it & 1) || p.odd_pairs)) tpend = j;
-                else __hip_atomic_fetch_add(P.T + j, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else __hip_atomic_fetch_add(PT + j, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               }
             }
             if (j >= 0) ndep += 1;
@@ -1049,7 +1056,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             ndep += 1;
             if (!ADJ) {
               if (LDSH) atomicAdd(shist + idx, 1u);
-              else __hip_atomic_fetch_add(P.E + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              else __hip_atomic_fetch_add(PE + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
               // score of the trajectory up to this observation: A = d log L / d log lam
               // (scattering tests), k = removal tests survived
@@ -1082,7 +1089,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
       }
     }
 
-    if (TMAJ && tpend >= 0) __hip_atomic_fetch_add(P.T + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (TMAJ && tpend >= 0) __hip_atomic_fetch_add(PT + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (RUNS3 && jrow >= 0) flush_run();
     __syncthreads();
     if (TMAJ && !ADJ) {
@@ -1098,17 +1105,17 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
           const uint32_t row = j / (uint32_t)p.nt_pad, t = j - row * (uint32_t)p.nt_pad;
           const uint32_t mode = row >= p.tsp ? 1u : 0u;
           const uint32_t sp = row - mode * p.tsp + (p.tcomp ? (uint32_t)bs.tdir * p.stride[3] : 0u);
-          __hip_atomic_fetch_add(P.E + ((int64_t)sp + (int64_t)t * p.time_stride + (mode ? p.mode_stride : 0)), c,
+          __hip_atomic_fetch_add(PE + ((int64_t)sp + (int64_t)t * p.time_stride + (mode ? p.mode_stride : 0)), c,
                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-          __hip_atomic_fetch_add(P.T + (v >> 24), c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(PT + (v >> 24), c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
     }
     if (LDSH && !ADJ) {
       for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) {
         uint32_t c = shist[k];
-        if (c) __hip_atomic_fetch_add(P.E + k, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c) __hip_atomic_fetch_add(PE + k, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     if (P.counters) {
