@@ -302,6 +302,8 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   int ncu = 0, max_smem = 0;
   RTMC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   RTMC_HIP(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  // what the plan may size: the work-group's LDS less the kernel's static part
+  max_smem -= kTraceStaticLds;
 
   size_t edge_bytes = (size_t)total * sizeof(double);
   bool edges_lds = edge_bytes <= 32 * 1024;
@@ -350,13 +352,16 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
               (r3 == 2 || (r3 == 1 && nsub * pl.sub_words * 8 <= ((int64_t)4 << 30)));
     if (p.runs3) p.odd_pairs = 0;
     p.run_n = (int)rp;
+    // 1: every run as a 64-bit atomic (a run of one otherwise goes out as a 32-bit
+    // atomic on the low half of its word)
+    p.run64 = getenv_int("RTMC_RUNS3_ATOMIC64", 0) != 0;
     p.run_words = pl.sub_words;
   }
   pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
   pl.tmaj_auto = tmaj && hist_mode == 0;
   if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
-    set_error("LDS %s of %zu bytes exceeds the %d bytes of LDS per work-group", adjoint ? "weight table" : "histogram",
-              hist_bytes, max_smem);
+    set_error("LDS %s of %zu bytes (with %zu bytes of bin edges) exceeds the %d bytes of LDS a work-group has "
+              "beside the kernel's own", adjoint ? "weight table" : "histogram", hist_bytes, edge_bytes, max_smem);
     return RTMC_ERR_ARG;
   }
   // The global guide (run_plan) replaces the LDS guide, which then takes no LDS; if
@@ -397,16 +402,17 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   for (int v = hot; v > 1; v >>= 1) p.hot_shift -= 1;
   p.hot_off = (int)((pl.smem + 7) & ~(size_t)7);
   if (hot) pl.smem = (size_t)p.hot_off + 8 * (size_t)hot;
-  RTMC_REQUIRE(pl.smem <= (size_t)max_smem, "%zu bytes of LDS per work-group exceed the %d available", pl.smem,
-               max_smem);
+  RTMC_REQUIRE(pl.smem <= (size_t)max_smem, "%zu bytes of LDS per work-group exceed the %d available beside the "
+               "kernel's own", pl.smem, max_smem);
 
   unsigned block = block_size > 0 ? (unsigned)block_size : (ldsh ? 1024u : 256u);
   RTMC_REQUIRE(block % 64 == 0 && block <= (ldsh ? 1024u : 256u),
                "block_size must be a multiple of 64 and <= 256 (<= 1024 with the LDS histogram)");
+  // resident work-groups per CU as LDS allows (the whole work-group's LDS, static part included)
+  const int64_t lds_cu = (int64_t)max_smem + kTraceStaticLds, lds_wg = (int64_t)pl.smem + kTraceStaticLds;
   int64_t per_cu;
-  if (ldsh) per_cu = std::max<int64_t>(1, (int64_t)max_smem / (int64_t)std::max<size_t>(pl.smem, 1));
-  else per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / block, pl.smem ? (int64_t)max_smem / (int64_t)pl.smem
-                                                                              : 2048 / block));
+  if (ldsh) per_cu = std::max<int64_t>(1, lds_cu / lds_wg);
+  else per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / block, lds_cu / lds_wg));
   RTMC_REQUIRE(ppt >= 0, "particles_per_thread must be >= 0");
   int64_t want = ppt > 0 ? (n_total + (int64_t)block * ppt - 1) / ((int64_t)block * ppt)
                          : (n_total + block - 1) / block;

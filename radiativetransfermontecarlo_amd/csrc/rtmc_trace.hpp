@@ -159,6 +159,7 @@ struct LaunchParams {
   uint32_t tsp;       // spatial cells per mode in the scratch (time_stride, or stride[3] when tcomp)
   int odd_pairs;      // pairs also start at odd times (into T2)
   int runs3;          // HM_TMAJOR: runs of up to three times per atomic (T holds 21-bit counters)
+  int run64;          // runs of three: a run of one as a 64-bit atomic too (else 32-bit on the low half)
   int tw;             // runs of three: words per row and shift, ceil(n_times / 3)
   int run_n;          // runs of three: particles per scratch copy (<= 2^21 - 1)
   int64_t run_words;  // runs of three: 64-bit words of one scratch copy
@@ -596,6 +597,13 @@ __device__ __forceinline__ int pick_problem(const LaunchParams& p, const ProbDev
 // LDS-histogram kernels run one 1024-lane work-group per CU (the histogram
 // takes most of the LDS); the others use 256-lane work-groups.
 // POS: the direction variable is the cosine to the position vector (RTMC_DIRVAR_POSITION)
+// LDS the tracing kernel declares statically, beside the dynamic block the plan
+// sizes: the bin set-up, the angle-table ranges and two work-sharing words
+// (with room for their alignment).
+constexpr int kTraceStaticLds = 256;
+static_assert(sizeof(BinSetup) + 8 * sizeof(double) + 2 * sizeof(int) + 32 <= (size_t)kTraceStaticLds,
+              "kTraceStaticLds must cover the kernel's static LDS");
+
 template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK, bool POS>
 __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const LaunchParams p,
                                                                      const ProbDev* __restrict__ probs) {
@@ -750,12 +758,15 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
       const unsigned w = tr / 3u, s = tr - 3u * w;
       RTMC_GLOBAL unsigned long long* a = (RTMC_GLOBAL unsigned long long*)PT + (size_t)(uint32_t)rsub * (size_t)p.run_words +
                               ((size_t)(uint32_t)jrow * 3u + s) * (size_t)(uint32_t)p.tw + w;
-      if (lrun == 1) {
+      if (lrun == 1 && !p.run64) {
         // the first counter (bits 0-20) alone: a 32-bit atomic on the low half, which
-        // the memory side executes faster than a 64-bit one (no carry leaves bit 20)
+        // the memory side executes faster than a 64-bit one.  No carry leaves bit 20
+        // (a copy serves fewer than 2^21 particles), and the device's atomics on the
+        // two halves of one aligned 64-bit word act on the same memory as 64-bit
+        // atomics on it, so mixing the two widths adds exactly.
         __hip_atomic_fetch_add((RTMC_GLOBAL uint32_t*)a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __hip_atomic_fetch_add(a, lrun == 2 ? 0x200001ull : 0x40000200001ull, __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(a, lrun == 1 ? 1ull : (lrun == 2 ? 0x200001ull : 0x40000200001ull), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
       }
       jrow = -1;

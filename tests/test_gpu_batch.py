@@ -473,6 +473,11 @@ def test_runs_of_three_match_pairs_and_direct_atomics(gpu, monkeypatch, tmax, ru
     E3, c3, _k = solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3)
     assert torch.equal(Ep, ref) and torch.equal(cp, rc)
     assert torch.equal(E3, ref) and torch.equal(c3, rc)
+    # a run of one as a 64-bit atomic too (instead of 32-bit on the word's low half)
+    monkeypatch.setenv("RTMC_RUNS3_ATOMIC64", "1")
+    E6, c6, _k = solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3)
+    monkeypatch.delenv("RTMC_RUNS3_ATOMIC64")
+    assert torch.equal(E6, ref) and torch.equal(c6, rc)
     # accumulation: a second call adds
     solver.forward_histogram(geometry, source, m, O, n, seed=37, device='cuda', hist_mode=3, E=E3)
     assert torch.equal(E3, 2 * ref)
