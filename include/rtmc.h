@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RTMC_ABI_VERSION 4
+#define RTMC_ABI_VERSION 5
 
 /* status codes */
 #define RTMC_OK 0
@@ -438,6 +438,8 @@ typedef struct {
     double knot_threshold;      /* 1e-8: the reference's tables; 0: every strictly increasing knot,
                                    held beyond the last one (what the adjoint's score assumes) */
     int64_t n_theta;            /* angle grid of the pdf; 0 = 1e6 as prepareSigmaOne */
+    double rho;                 /* density; > 0 (elastic): rtmc_prepare_material[s] also fill the
+                                   solid/air Zoeppritz tables (n_zoe = 181); 0 = none */
 } rtmc_material_params;
 
 /* Host-side result of rtmc_prepare_material_host (arrays malloc'ed by the
@@ -461,8 +463,9 @@ void rtmc_free_material_host(rtmc_material_host* out);
 
 /* The same, uploaded: fills `out` (device tables) for rtmc_forward and the
  * batch entry points; *handle owns the device memory until
- * rtmc_free_material.  Zoeppritz tables (elastic reflective boundaries) are
- * not computed here (out->n_zoe = 0). */
+ * rtmc_free_material.  An elastic material with params->rho > 0 also gets
+ * the Zoeppritz tables of elastic reflective boundaries
+ * (rtmc_zoeppritz_air on 181 angles); otherwise out->n_zoe = 0. */
 int rtmc_prepare_material(const rtmc_material_params* params, rtmc_material* out, void** handle);
 int rtmc_free_material(void* handle);
 
@@ -472,6 +475,15 @@ int rtmc_free_material(void* handle);
  * (rtmc_free_material).  Errors name the first failing material. */
 int rtmc_prepare_materials(const rtmc_material_params* params, int32_t n, rtmc_material* out,
                            void** handle);
+
+/* Replaces MaterialClass.Zoeppritz (MaterialClass.m:1786-1830) for its one
+ * case, a solid under air (vp 343, vs 0, rho 1; ZoeppritzFluid.m:1-245):
+ * at the n incidence angles deg = linspace(0, 90, n) (the reference uses 181),
+ * the reflected energy fractions E_Rpp (P incidence) and E_Rsvsv (SV
+ * incidence) of a solid with velocities vp, vs and density rho.  Host arrays
+ * of n values each. */
+int rtmc_zoeppritz_air(double vp, double vs, double rho, int64_t n, double* deg, double* E_Rpp,
+                       double* E_Rsvsv);
 
 /* No counterpart in the reference: the material derivatives the adjoint
  * (rtmc_adjoint_batched) needs, by central differences of calcSigmaIsotropic

@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK = 0
 ERR_ARG = -1
@@ -258,6 +258,7 @@ class MaterialParams(C.Structure):
         ("Q", C.c_double * 2),
         ("knot_threshold", C.c_double),
         ("n_theta", C.c_int64),
+        ("rho", C.c_double),
     ]
 
 
@@ -308,6 +309,7 @@ EXPORTS = (
     "rtmc_free_material",
     "rtmc_prepare_materials",
     "rtmc_lc_derivatives",
+    "rtmc_zoeppritz_air",
     "rtmc_probe_atomics",
     "rtmc_set_trace_timing",
     "rtmc_trace_times",
@@ -359,6 +361,9 @@ def _declare(lib):
     lib.rtmc_lc_derivatives.restype = C.c_int
     lib.rtmc_lc_derivatives.argtypes = [P(MaterialParams), C.c_int32, C.c_int64, C.c_double, P(C.c_double),
                                         P(C.c_double)]
+    lib.rtmc_zoeppritz_air.restype = C.c_int
+    lib.rtmc_zoeppritz_air.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int64, P(C.c_double),
+                                       P(C.c_double), P(C.c_double)]
     lib.rtmc_set_trace_timing.restype = C.c_int
     lib.rtmc_set_trace_timing.argtypes = [C.c_int32]
     lib.rtmc_trace_times.restype = C.c_int

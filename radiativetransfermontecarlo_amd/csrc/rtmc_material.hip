@@ -2,12 +2,15 @@
 // MaterialClass (cross-sections, total cross-sections, inverse-CDF tables of
 // the scattering angle) computed natively on the host and uploaded for the
 // kernels.  Restates MaterialClass.m:172-359 (calcSigmaIsotropic),
-// :381-710 (PSDF laws), :2379-2463 (prepareSigma, prepareSigmaOne).
+// :381-710 (PSDF laws), :2379-2463 (prepareSigma, prepareSigmaOne), and the
+// solid/air energy coefficients of elastic reflective boundaries
+// (MaterialClass.m:1786-1830 with ZoeppritzFluid.m).
 #include "rtmc_common.hpp"
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -261,6 +264,91 @@ struct DeviceTables {
   std::vector<void*> ptrs;
 };
 
+// ---- solid/air interface (MaterialClass.Zoeppritz, MaterialClass.m:1786-1830) ----
+// The fluid is air (vp 343, vs 0, rho 1, :1799-1801) and the incidence angles
+// linspace(0, 90, n) degrees (:1804).  For each angle ZoeppritzFluid.m solves the
+// boundary conditions for P incidence (:49-121) and SV incidence (:123-188): the
+// three equations left when the fluid carries no S wave, in the unknowns
+// (reflected P, reflected SV, transmitted P) amplitudes.  Angles past a critical
+// one are complex (asin of a slowness above 1), so the system is complex.
+constexpr double kAirVp = 343.0, kAirVs = 0.0, kAirRho = 1.0;
+constexpr int kZoeAngles = 181;
+using cplx = std::complex<double>;
+
+cplx casin(double v) { return std::asin(cplx(v, 0.0)); }
+
+// Solves the 3x3 system M a = f by elimination with partial pivoting; false if singular.
+bool solve3(cplx M[3][3], cplx f[3], cplx a[3]) {
+  for (int c = 0; c < 3; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 3; ++r)
+      if (std::abs(M[r][c]) > std::abs(M[piv][c])) piv = r;
+    if (std::abs(M[piv][c]) == 0.0) return false;
+    if (piv != c) {
+      for (int k = 0; k < 3; ++k) std::swap(M[c][k], M[piv][k]);
+      std::swap(f[c], f[piv]);
+    }
+    for (int r = c + 1; r < 3; ++r) {
+      const cplx m = M[r][c] / M[c][c];
+      for (int k = c; k < 3; ++k) M[r][k] -= m * M[c][k];
+      f[r] -= m * f[c];
+    }
+  }
+  for (int r = 2; r >= 0; --r) {
+    cplx acc = f[r];
+    for (int k = r + 1; k < 3; ++k) acc -= M[r][k] * a[k];
+    a[r] = acc / M[r][r];
+  }
+  return true;
+}
+
+// Rows of ZoeppritzFluid.m's boundary conditions that involve no fluid S wave,
+// for angles phi1 (reflected P), psi1 (reflected S), phi2 (transmitted P),
+// psi2 (transmitted S, zero slowness in the fluid).
+void interface_matrix(double vp1, double vs1, double rho1, cplx phi1, cplx psi1, cplx phi2, cplx psi2,
+                      cplx M[3][3]) {
+  const double vp2 = kAirVp, vs2 = kAirVs, rho2 = kAirRho;
+  M[0][0] = std::cos(phi1);
+  M[0][1] = -std::sin(psi1);
+  M[0][2] = std::cos(phi2);
+  M[1][0] = -std::cos(2.0 * psi1);
+  M[1][1] = vs1 / vp1 * std::sin(2.0 * psi1);
+  M[1][2] = (rho2 * vp2 / (rho1 * vp1)) * std::cos(2.0 * psi2);
+  M[2][0] = std::sin(2.0 * phi1);
+  M[2][1] = vp1 / vs1 * std::cos(2.0 * psi1);
+  M[2][2] = (rho2 * vp1 * vs2 * vs2 / (rho1 * vp2 * vs1 * vs1)) * std::sin(2.0 * phi2);
+}
+
+// E_Rpp (P incidence, reflected P energy) and E_Rsvsv (SV incidence, reflected
+// SV energy) at n angles linspace(0, 90, n); NaN where a system is singular.
+void zoeppritz_air(double vp1, double vs1, double rho1, int64_t n, double* deg, double* Rpp, double* Rsvsv) {
+  const double vp2 = kAirVp, vs2 = kAirVs;
+  for (int64_t i = 0; i < n; ++i) {
+    // linspace(0, 90, n) as MATLAB forms it: d1 + (0:n-1) * (d2 - d1) / (n - 1), last = d2
+    const double th = (i == n - 1) ? 90.0 : (double)i * (90.0 / (double)(n - 1));
+    deg[i] = th;
+    const double t = th * kPiH / 180.0;
+    cplx M[3][3], f[3], a[3];
+    {  // P incidence
+      const double p = std::sin(t) / vp1;
+      interface_matrix(vp1, vs1, rho1, cplx(t, 0.0), casin(vs1 * p), casin(vp2 * p), casin(vs2 * p), M);
+      const cplx psi0 = casin(vs1 * std::sin(t) / vp1);
+      f[0] = std::cos(t);
+      f[1] = std::cos(2.0 * psi0);
+      f[2] = std::sin(2.0 * t);
+      Rpp[i] = solve3(M, f, a) ? std::norm(a[0]) : NAN;
+    }
+    {  // SV incidence
+      const double p = std::sin(t) / vs1;
+      interface_matrix(vp1, vs1, rho1, casin(vp1 * p), cplx(t, 0.0), casin(vp2 * p), casin(vs2 * p), M);
+      f[0] = -std::sin(t);
+      f[1] = -vs1 * std::sin(2.0 * t) / vp1;
+      f[2] = vp1 * std::cos(2.0 * t) / vs1;
+      Rsvsv[i] = solve3(M, f, a) ? std::norm(a[1]) : NAN;
+    }
+  }
+}
+
 }  // namespace
 }  // namespace rtmc
 
@@ -314,6 +402,7 @@ int check_params(const rtmc_material_params* p) {
       RTMC_REQUIRE(std::fabs(p->corr[k]) <= 1.0, "Absolute values of correlation coefficients should be less than 1!");
   }
   RTMC_REQUIRE(p->n_theta == 0 || p->n_theta >= 16, "n_theta must be >= 16");
+  RTMC_REQUIRE(p->rho >= 0.0 && std::isfinite(p->rho), "rho must be >= 0 and finite (0: no Zoeppritz tables)");
   return RTMC_OK;
 }
 
@@ -467,9 +556,25 @@ extern "C" int rtmc_prepare_material(const rtmc_material_params* p, rtmc_materia
     out->invcdf[c].extrap = h.extrap;
   }
   rtmc_free_material_host(&h);
+  if (rc == RTMC_OK && !p->acoustics && p->rho > 0.0) {
+    // solid/air energy coefficients for elastic reflective boundaries
+    double zh[3 * kZoeAngles];
+    zoeppritz_air(p->v[0], p->v[1], p->rho, kZoeAngles, zh, zh + kZoeAngles, zh + 2 * kZoeAngles);
+    void* dz = nullptr;
+    if (hipMalloc(&dz, sizeof(zh)) != hipSuccess || hipMemcpy(dz, zh, sizeof(zh), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("uploading the Zoeppritz tables failed");
+      rc = RTMC_ERR_NOMEM;
+    }
+    if (dz) dt->ptrs.push_back(dz);
+    out->n_zoe = kZoeAngles;
+    out->zoe_deg = (const double*)dz;
+    out->zoe_Rpp = (const double*)dz + kZoeAngles;
+    out->zoe_Rsvsv = (const double*)dz + 2 * kZoeAngles;
+  }
   if (rc != RTMC_OK) {
     for (void* q : dt->ptrs) (void)hipFree(q);
     delete dt;
+    memset(out, 0, sizeof(*out));
     return rc;
   }
   *handle = dt;
@@ -499,6 +604,7 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
   for (int i = 0; i < n; ++i) {
     const int nch = params[i].acoustics ? 1 : 4;
     for (int c = 0; c < nch; ++c) total += 2 * (size_t)h[(size_t)i].n_knots[c];
+    if (!params[i].acoustics && params[i].rho > 0.0) total += 3 * (size_t)kZoeAngles;
   }
   void* dev = nullptr;
   if (hipMalloc(&dev, total * sizeof(double)) != hipSuccess) {
@@ -533,6 +639,16 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
       m.invcdf[c].y = (const double*)dev + off + k;
       m.invcdf[c].extrap = hi.extrap;
       off += 2 * k;
+    }
+    if (!p->acoustics && p->rho > 0.0) {
+      double zh[3 * kZoeAngles];
+      zoeppritz_air(p->v[0], p->v[1], p->rho, kZoeAngles, zh, zh + kZoeAngles, zh + 2 * kZoeAngles);
+      copied = copied && hipMemcpy((double*)dev + off, zh, sizeof(zh), hipMemcpyHostToDevice) == hipSuccess;
+      m.n_zoe = kZoeAngles;
+      m.zoe_deg = (const double*)dev + off;
+      m.zoe_Rpp = (const double*)dev + off + kZoeAngles;
+      m.zoe_Rsvsv = (const double*)dev + off + 2 * kZoeAngles;
+      off += 3 * (size_t)kZoeAngles;
     }
   }
   free_all();
@@ -586,6 +702,17 @@ extern "C" int rtmc_lc_derivatives(const rtmc_material_params* params, int32_t n
     dlogSigma_dlc[i] = (std::log(Sp_) - std::log(Sm_)) / (2.0 * h);
     return RTMC_OK;
   });
+}
+
+extern "C" int rtmc_zoeppritz_air(double vp, double vs, double rho, int64_t n, double* deg, double* E_Rpp,
+                                  double* E_Rsvsv) {
+  clear_error();
+  RTMC_REQUIRE(deg != nullptr && E_Rpp != nullptr && E_Rsvsv != nullptr, "outputs must be given");
+  RTMC_REQUIRE(n >= 2, "n must be >= 2");
+  RTMC_REQUIRE(vp > 0.0 && vs > 0.0 && rho > 0.0 && std::isfinite(vp) && std::isfinite(vs) && std::isfinite(rho),
+               "vp, vs and rho must be positive and finite");
+  zoeppritz_air(vp, vs, rho, n, deg, E_Rpp, E_Rsvsv);
+  return RTMC_OK;
 }
 
 extern "C" int rtmc_free_material(void* handle) {

@@ -3,7 +3,8 @@ computed in librtmc (C++ on the host), for isotropic random media.
 
 ``params_from(material, d)`` maps a MaterialClass to ``rtmc_material_params``;
 ``prepare_host`` returns Sigma, Sigma', the mean free times, P2P / S2S and the
-inverse-CDF tables as numpy arrays; ``NativeMaterial`` uploads them for the
+inverse-CDF tables as numpy arrays; ``NativeMaterial`` uploads them (and, for an
+elastic material with a density, the solid/air Zoeppritz tables) for the
 kernels (an ``rtmc_material`` usable wherever ``DeviceMaterial.c`` is).
 """
 from __future__ import annotations
@@ -16,7 +17,8 @@ import numpy as np
 from . import _abi
 from .material import MaterialClass
 
-__all__ = ["params_from", "params_for", "prepare_host", "NativeMaterial", "NativeMaterials", "lc_derivatives"]
+__all__ = ["params_from", "params_for", "prepare_host", "NativeMaterial", "NativeMaterials", "lc_derivatives",
+           "zoeppritz_air"]
 
 
 def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n_theta: int = 0):
@@ -54,7 +56,21 @@ def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n
     p.Q[1] = Q[1] if Q.size > 1 else Q[0]
     p.knot_threshold = float(knot_threshold)
     p.n_theta = int(n_theta)
+    # an elastic density gives the solid/air tables of reflective boundaries
+    rho = getattr(material, 'rho', None)
+    p.rho = float(rho) if (rho is not None and not material.acoustics) else 0.0
     return p
+
+
+def zoeppritz_air(vp: float, vs: float, rho: float, n: int = 181) -> dict:
+    """rtmc_zoeppritz_air: E_Rpp and E_Rsvsv of a solid under air at
+    linspace(0, 90, n) degrees (MaterialClass.Zoeppritz)."""
+    lib = _abi.load()
+    deg, rpp, rss = (np.zeros(n) for _ in range(3))
+    P = C.POINTER(C.c_double)
+    _abi.check(lib.rtmc_zoeppritz_air(float(vp), float(vs), float(rho), int(n), deg.ctypes.data_as(P),
+                                      rpp.ctypes.data_as(P), rss.ctypes.data_as(P)), "rtmc_zoeppritz_air")
+    return {'j1_deg': deg, 'E_Rpp': rpp, 'E_Rsvsv': rss}
 
 
 def params_for(base: MaterialClass, d: int, c: float, lc: float, Q: float, knot_threshold: float = 1e-8,

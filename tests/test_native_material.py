@@ -81,6 +81,37 @@ def test_native_rejects_bad_input():
         nm.prepare_host(p)
 
 
+def test_native_zoeppritz_matches_port_and_closed_forms():
+    """rtmc_zoeppritz_air (solid under air, MaterialClass.Zoeppritz) against the
+    host restatement of ZoeppritzFluid.m (itself pinned by energy balance in
+    tests/test_material.py) to 1e-13, the angle grid linspace(0, 90, 181)
+    exactly, and at normal incidence the closed forms E_Rpp = ((Z2 - Z1) /
+    (Z2 + Z1))^2 (impedances rho v) and E_Rsvsv = 1 (no S wave in air)."""
+    from radiativetransfermontecarlo_amd.material import zoeppritz_fluid
+    th = np.linspace(0, 90, 181)
+    for vp, vs, rho in ((2500.0, 2500.0 / math.sqrt(3.0), 2000.0), (6.0, 6.0 / math.sqrt(3.0), 2000.0),
+                        (5000.0, 2000.0, 2700.0)):
+        got = nm.zoeppritz_air(vp, vs, rho)
+        ref = zoeppritz_fluid(th, vp, vs, rho, 343.0, 0.0, 1.0)
+        assert np.array_equal(got['j1_deg'], th)
+        for k in ('E_Rpp', 'E_Rsvsv'):
+            assert np.max(np.abs(got[k] - ref[k])) < 1e-13, k
+        Z1, Z2 = rho * vp, 343.0
+        assert got['E_Rpp'][0] == pytest.approx(((Z2 - Z1) / (Z2 + Z1)) ** 2, rel=1e-13)
+        assert got['E_Rsvsv'][0] == pytest.approx(1.0, abs=1e-13)
+    with pytest.raises(Exception, match="positive and finite"):
+        nm.zoeppritz_air(2500.0, 0.0, 2000.0)
+
+
+def test_native_params_carry_density_for_elastic_media():
+    assert nm.params_from(cases.config3()[2], 3).rho == 2000.0
+    assert nm.params_from(cases.config1()[2], 3).rho == 0.0
+    p = nm.params_from(cases.config3()[2], 3)
+    p.rho = -1.0
+    with pytest.raises(Exception, match="rho must be >= 0"):
+        nm.prepare_host(p)
+
+
 def _forward_pair(geometry, source, material, observation, n, seed=5):
     from radiativetransfermontecarlo_amd import solver
     from radiativetransfermontecarlo_amd.observation import initializeObservation
@@ -98,14 +129,19 @@ def _forward_pair(geometry, source, material, observation, n, seed=5):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["acoustic 3-D exp", "elastic 2-D gaussian"])
+@pytest.mark.parametrize("which", ["acoustic 3-D exp", "elastic 2-D gaussian", "elastic 3-D half-space"])
 def test_native_tables_drive_the_kernel_like_host_tables(gpu, which):
     """rtmc_prepare_material's device tables in rtmc_forward: with the same seed
     the histogram equals the one from the host-prepared tables except where a
-    position falls within the tables' ~1e-12 disagreement of a bin edge."""
+    position falls within the tables' ~1e-12 disagreement of a bin edge.  The
+    half-space (a free surface z = 0) also reads the native Zoeppritz tables."""
     if which.startswith("acoustic"):
         geometry, source, material, observation = cases.config1()
         observation = dict(observation, time=np.arange(0, 2 + 1e-9, 0.02))
+        n = 200_000
+    elif which.endswith("half-space"):
+        geometry, source, material, observation = cases.config3()
+        observation = dict(observation, time=np.arange(0, 3 + 1e-9, 0.05))
         n = 200_000
     else:
         geometry, source, material, observation = cases.config2()
@@ -114,6 +150,8 @@ def test_native_tables_drive_the_kernel_like_host_tables(gpu, which):
         n = 100_000
     a, b, ca, cb = _forward_pair(geometry, source, material, observation, n)
     assert a.sum() > 0
+    if which.endswith("half-space"):
+        assert ca[2] > 0 and abs(int(ca[2]) - int(cb[2])) <= 1e-5 * ca[2]   # boundary interactions
     assert a.sum() == b.sum() or abs(int(a.sum()) - int(b.sum())) <= 1e-6 * a.sum()
     diff = np.abs(a - b).sum()
     assert diff <= 1e-5 * a.sum(), (diff, a.sum())
@@ -165,6 +203,7 @@ def test_native_batched_preparation_equals_single(gpu):
     g3, s3, base3, o3, params, Ns = cases.config4()
     plist = [nm.params_for(base3, 3, *params[i], knot_threshold=0.0) for i in range(6)]
     plist.append(nm.params_from(cases.config2()[2], 2))
+    plist.append(nm.params_from(cases.config3()[2], 3))      # with a density: Zoeppritz tables
     many = nm.NativeMaterials(plist)
 
     for i, p in enumerate(plist):
@@ -179,5 +218,11 @@ def test_native_batched_preparation_equals_single(gpu):
                 ha = _download(pa, ta.n)
                 hb = _download(pb, tb.n)
                 assert np.array_equal(ha, hb), (i, c)
+        assert a.n_zoe == b.n_zoe == (181 if p.rho > 0 else 0)
+        if a.n_zoe:
+            z = nm.zoeppritz_air(p.v[0], p.v[1], p.rho)
+            for pa, pb, key in ((a.zoe_deg, b.zoe_deg, 'j1_deg'), (a.zoe_Rpp, b.zoe_Rpp, 'E_Rpp'),
+                                (a.zoe_Rsvsv, b.zoe_Rsvsv, 'E_Rsvsv')):
+                assert np.array_equal(_download(pa, 181), z[key]) and np.array_equal(_download(pb, 181), z[key])
         one.close()
     many.close()
