@@ -48,11 +48,17 @@ FLOP_PER_DEPOSIT = {"radial": 24, "cartesian": 16}
 FLOP_PER_BOUNDARY = 30
 # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries (bytes per launch of the tracing kernel)
 TRAFFIC = {(5, "fp64"): "k_forward_acoustic3d_1e9_fp64_traffic.json",
+           (5, "fp64_lds"): "k_forward_acoustic3d_1e9_fp64_lds_traffic.json",
            (2, "fp64"): "k_forward_elastic2d_1e7_traffic.json",
            (3, "fp64"): "k_forward_elastic3d_halfspace_5e7_traffic.json"}
 FP32_PER_BIN = ("FP32 flight vs FP64 on a 31^3 x 8 x 101 version of this workload, 1e7 particles each: "
                 "0.27 % of 875 279 populated bins beyond 3 sigma, max |z| 5.0, mean z^2 1.002 "
                 "(tests/test_gpu_forward.py::test_fp32_flight_per_bin_error_against_fp64)")
+
+
+LDS_TOL = 1e-5
+LDS_PER_BIN = ("compressed tables (tol 1e-5 rad) vs full tables on a 31^3 x 8 x 101 version of this workload, 1e7 "
+               "particles each (tests/test_gpu_lds_table.py::test_lds_table_per_bin_error_against_full_table)")
 
 
 def parse():
@@ -85,6 +91,10 @@ def parse():
                     help="only measure the global-atomic request rate (rtmc_probe_atomics) over several ranges")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="flight arithmetic of the fused kernel (fp32: FP32 flight, FP64 time)")
+    ap.add_argument("--lds-tol", type=float, default=0.0,
+                    help="> 0: draw scattering angles from compressed tables held in LDS, within this many "
+                         "radians of the full tables (0: the full tables in global memory)")
+    ap.add_argument("--lds-max-knots", type=int, default=1024)
     return ap.parse_args()
 
 
@@ -111,7 +121,8 @@ WORKLOADS = {
 DEFAULT_N = {1: 1_000_000, 2: 10_000_000, 3: 50_000_000, 5: 1_000_000_000}
 
 
-def measure(config, precision, steps, warmup, particles=0, chunks=1, block=0, ppt=0, quiet=False):
+def measure(config, precision, steps, warmup, particles=0, chunks=1, block=0, ppt=0, quiet=False, lds_tol=0.0,
+            lds_max_knots=1024):
     """Times `steps` full forward solves of a workload; returns the figures and
     the problem (for the CPU baseline)."""
     import ctypes as C
@@ -133,7 +144,11 @@ def measure(config, precision, steps, warmup, particles=0, chunks=1, block=0, pp
     d = geometry['dimension']
     material = MaterialClass.prepareSigma(material, d)
     O = initializeObservation(geometry, bool(material.acoustics), observation, N, allow_nd=True)
-    dmat = solver.DeviceMaterial(material, d, dev, with_zoeppritz=len(geometry.get('bnd', [])) > 0)
+    dmat = solver.DeviceMaterial(material, d, dev, with_zoeppritz=len(geometry.get('bnd', [])) > 0, lds_tol=lds_tol,
+                                 lds_max_knots=lds_max_knots)
+    if lds_tol > 0:
+        workload += (f", angles from compressed tables in LDS ({dmat.c.lds[0].m} knots, "
+                     f"within {dmat.c.lds[0].tol:g} rad of the full table)")
     dbins = solver.DeviceBins(O, dev)
     bounds = [N * k // chunks for k in range(chunks + 1)]
     E = torch.zeros(O.size, dtype=torch.int32, device=dev)
@@ -184,15 +199,16 @@ def measure(config, precision, steps, warmup, particles=0, chunks=1, block=0, pp
     return out, prob
 
 
-def roofline(m, config, precision):
-    """Dominant kernel against its bound; HBM (histogram deposits) or FP64 VALU."""
+def roofline(m, config, precision, variant=""):
+    """Dominant kernel against its bound; HBM (histogram deposits) or FP64 VALU.
+    variant: suffix of the traffic summary's key ("_lds": compressed angle tables)."""
     if config in (2, 5):
         alg = BYTES_PER_DEPOSIT * m["deposits"]
         k_gbs = alg / (m["kernel_ms"] * 1e-3) / 1e9
         s_bytes = alg + 2 * m["histogram_bytes"]
         s_gbs = s_bytes / (m["ms_per_step"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": k_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(config, precision),
+                "frac": k_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(config, precision + variant),
                 "kernel": "k_trace", "kernel_ms": m["kernel_ms"],
                 "algorithmic_unit": "kernel-only: 8 B per histogram deposit (4 B read + 4 B write of the counter) "
                                     "x deposits per launch, over the kernel's duration",
@@ -206,7 +222,7 @@ def roofline(m, config, precision):
     tf = flops / (m["kernel_ms"] * 1e-3) / 1e12
     peak = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
     out = {"bound": "fp64-valu" if precision == "fp64" else "fp32-valu", "achieved": tf, "peak": peak,
-           "unit": "TFLOP/s", "frac": tf / peak, "traffic": traffic_from_profiles(config, precision),
+           "unit": "TFLOP/s", "frac": tf / peak, "traffic": traffic_from_profiles(config, precision + variant),
            "kernel": "k_trace", "kernel_ms": m["kernel_ms"],
            "algorithmic_unit": f"{FLOP_PER_EVENT[key]} flop-eq per scattering event + {fobs} per deposit + "
                                f"{FLOP_PER_BOUNDARY} per boundary hit"}
@@ -297,7 +313,7 @@ def main():
         print(json.dumps({"copy_bytes": n * 4}))
         return
     m, prob = measure(args.config, args.precision, args.steps, args.warmup, args.particles, max(1, args.chunks),
-                      args.block, args.ppt)
+                      args.block, args.ppt, lds_tol=args.lds_tol, lds_max_knots=args.lds_max_knots)
     out = {
         "metric": "scattering events/s on 1 MI355X",
         "value": m["value"],
@@ -314,7 +330,7 @@ def main():
         "config": {"workload": m["workload"], "particles": m["particles"], "histogram_bytes": m["histogram_bytes"],
                    "deposits_per_step": m["deposits"], "scatter_events_per_step": m["scatter_events"],
                    "deposits_per_s": m["deposits"] / (m["ms_per_step"] * 1e-3)},
-        "roofline": roofline(m, args.config, args.precision),
+        "roofline": roofline(m, args.config, args.precision, "_lds" if args.lds_tol else ""),
     }
     extras = args.config == 5 and not args.no_extras and not args.particles
     if extras and args.precision == "fp64":
@@ -323,6 +339,13 @@ def main():
                               "kernel_ms": f["kernel_ms"], "deposits_per_step": f["deposits"],
                               "scatter_events_per_step": f["scatter_events"], "roofline": roofline(f, 5, "fp32"),
                               "per_bin_error_vs_fp64": FP32_PER_BIN}
+    if extras and args.precision == "fp64" and not args.lds_tol:
+        t, _ = measure(5, "fp64", args.steps, args.warmup, lds_tol=LDS_TOL)
+        out["lds_angle_table"] = {"workload": t["workload"], "value": t["value"], "unit": "scattering events/s",
+                                  "ms_per_step": t["ms_per_step"], "kernel_ms": t["kernel_ms"],
+                                  "deposits_per_step": t["deposits"], "scatter_events_per_step": t["scatter_events"],
+                                  "roofline": roofline(t, 5, "fp64", "_lds"),
+                                  "per_bin_error_vs_full_tables": LDS_PER_BIN}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, m, prob)
     if extras:
@@ -389,8 +412,9 @@ def run_batch_adjoint(args, emit=True):
 
     # target: perturbed parameters, independent particles (not timed)
     pt = params * np.array([1.05, 0.95, 1.10])
+    LT = dict(lds_tol=args.lds_tol, lds_max_knots=args.lds_max_knots)  # compressed angle tables (0: none)
     Bt = rb.Batch.from_params(geometry, O, base, pt, source, list(Ns), [7_000 + i for i in range(n)], device=dev,
-                              knot_threshold=KT)
+                              knot_threshold=KT, **LT)
     Bt.forward()
     target = Bt.energy_density()
 
@@ -410,7 +434,8 @@ def run_batch_adjoint(args, emit=True):
     # ---- batched forward + adjoint -------------------------------------
     sync()
     t0 = time.perf_counter()
-    B = rb.Batch.from_params(geometry, O, base, params, source, list(Ns), seeds, device=dev, knot_threshold=KT)
+    B = rb.Batch.from_params(geometry, O, base, params, source, list(Ns), seeds, device=dev, knot_threshold=KT,
+                             **LT)
     tabs = B.derivative_tables()
     t_prep_a = time.perf_counter() - t0
     tuning = B.autotune(0.02) if (args.autotune or not emit) else None
@@ -452,12 +477,12 @@ def run_batch_adjoint(args, emit=True):
     tp = time.perf_counter()
     plist = []
     for i in range(n):
-        plist.append(nm.params_for(base, 3, *params[i], knot_threshold=KT))
+        plist.append(nm.params_for(base, 3, *params[i], knot_threshold=KT, **LT))
         for k in range(3):
             for s in (1.0, -1.0):
                 p = params[i].copy()
                 p[k] += s * H[k] * params[i, k]
-                plist.append(nm.params_for(base, 3, *p, knot_threshold=KT))
+                plist.append(nm.params_for(base, 3, *p, knot_threshold=KT, **LT))
     nat = nm.NativeMaterials(plist)
     t_prep_b = time.perf_counter() - tp
     dbins = solver.DeviceBins(O, dev)
@@ -540,7 +565,9 @@ def run_batch_adjoint(args, emit=True):
                            "(log E - log E_target)^2 (without a count threshold a bin counted in the forward can be "
                            "empty in a perturbed solve and the finite difference is undefined); target at "
                            "(+5% c, -5% lc, +10% Q), independent seed",
-                   "fd_steps_rel": H.tolist(), "tables": "knot_threshold=0"},
+                   "fd_steps_rel": H.tolist(),
+                   "tables": "knot_threshold=0" + (f", compressed in LDS within {args.lds_tol:g} rad" if args.lds_tol
+                                                   else "")},
         "batched": {"wall_s": t_a + t_prep_a, "prep_s": t_prep_a,
                     "preparation": "rtmc_prepare_materials + rtmc_lc_derivatives (64 materials at once)",
                     "solve_s": t_a, "forward_ms": ms_fwd, "adjoint_ms": ms_adj, "forward_kernel_ms": k_fwd,

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RTMC_ABI_VERSION 5
+#define RTMC_ABI_VERSION 6
 
 /* status codes */
 #define RTMC_OK 0
@@ -101,6 +101,26 @@ typedef struct {
     int32_t pad_;
 } rtmc_table;
 
+/* One knot of a compressed inverse-CDF table (rtmc_compress_table):
+ * theta(q) = theta + (q - u) * slope for u <= q < the next knot's u. */
+typedef struct {
+    double u;                   /* cdf value of the knot (one of the table's own x) */
+    float theta;                /* the table's angle there */
+    float slope;                /* d theta / d u up to the next knot */
+} rtmc_lds_knot;
+
+/* No counterpart in the reference: an inverse-CDF table in the form the
+ * tracing kernel holds in LDS.  m of the table's own knots, chosen so that
+ * |theta(q) - table(q)| <= tol for every q (plus float rounding of theta and
+ * slope, <= 4e-7); the first and the last two knots of the table are kept,
+ * so the linear extrapolation beyond them is the table's own. */
+typedef struct {
+    int32_t m;                  /* knots (0 = none: the kernel reads the full table from global memory) */
+    int32_t extrap;             /* the table's RTMC_EXTRAP_* (NEAREST: q is clamped to the end knots) */
+    double tol;                 /* the bound above, in radians */
+    const rtmc_lds_knot* knots; /* device, m */
+} rtmc_lds_table;
+
 /* Prepared material: the fields of MaterialClass that the particle kernels
  * read after MaterialClass.prepareSigma (MaterialClass.m:2379-2442). */
 typedef struct {
@@ -122,6 +142,9 @@ typedef struct {
     const double* zoe_deg;      /* device: incidence angles in degrees */
     const double* zoe_Rpp;      /* device: E_Rpp */
     const double* zoe_Rsvsv;    /* device: E_Rsvsv */
+    /* Optional compressed copies of invcdf for the tracing kernel
+     * (rtmc_attach_lds_tables); m = 0 leaves every draw on invcdf itself. */
+    rtmc_lds_table lds[4];
 } rtmc_material;
 
 /* Source description (initializeParticle.m:3-160, README.md:180-198). */
@@ -440,6 +463,10 @@ typedef struct {
     int64_t n_theta;            /* angle grid of the pdf; 0 = 1e6 as prepareSigmaOne */
     double rho;                 /* density; > 0 (elastic): rtmc_prepare_material[s] also fill the
                                    solid/air Zoeppritz tables (n_zoe = 181); 0 = none */
+    double lds_tol;             /* > 0: rtmc_prepare_material[s] also fill lds[] (rtmc_compress_table
+                                   with this tol and lds_max_knots); 0 = none */
+    int32_t lds_max_knots;      /* 0 = 1024 */
+    int32_t pad2_;
 } rtmc_material_params;
 
 /* Host-side result of rtmc_prepare_material_host (arrays malloc'ed by the
@@ -484,6 +511,27 @@ int rtmc_prepare_materials(const rtmc_material_params* params, int32_t n, rtmc_m
  * of n values each. */
 int rtmc_zoeppritz_air(double vp, double vs, double rho, int64_t n, double* deg, double* E_Rpp,
                        double* E_Rsvsv);
+
+/* No counterpart in the reference: compresses one inverse-CDF table
+ * (griddedInterpolant(x, y) with linear extrapolation, MaterialClass.m:2459-2462;
+ * host arrays) into at most max_knots of its own knots for LDS.  Knots 0, 1,
+ * n-2 and n-1 are always kept; between them, a run of knots is dropped while
+ * the chord across it stays within tol of every dropped knot (both functions
+ * are piecewise linear, so then |theta(q) - table(q)| <= tol for every q).
+ * When more than max_knots are needed, tol is doubled until they fit;
+ * *tol_used is the bound achieved.  out: host array of max_knots. */
+int rtmc_compress_table(const double* x, const double* y, int64_t n, double tol, int32_t max_knots,
+                        rtmc_lds_knot* out, int32_t* m, double* tol_used);
+
+/* No counterpart in the reference: fills mats[i].lds[t] for every inverse-CDF
+ * table of n prepared materials (device tables, read back on `stream`), by
+ * rtmc_compress_table(tol, max_knots), one material per host thread; the
+ * knots live in one device block owned by *handle (rtmc_free_material).  The
+ * tracing kernel then holds them in LDS and draws every scattering angle
+ * from them, when they fit beside the launch's histogram (else from invcdf;
+ * RTMC_LDS_TABLE=0 forces invcdf). */
+int rtmc_attach_lds_tables(rtmc_material* mats, int32_t n, double tol, int32_t max_knots, void** handle,
+                           void* stream);
 
 /* No counterpart in the reference: the material derivatives the adjoint
  * (rtmc_adjoint_batched) needs, by central differences of calcSigmaIsotropic

@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OK = 0
 ERR_ARG = -1
@@ -96,6 +96,14 @@ class Table(C.Structure):
     ]
 
 
+class LdsKnot(C.Structure):
+    _fields_ = [("u", C.c_double), ("theta", C.c_float), ("slope", C.c_float)]
+
+
+class LdsTable(C.Structure):
+    _fields_ = [("m", C.c_int32), ("extrap", C.c_int32), ("tol", C.c_double), ("knots", C.c_void_p)]
+
+
 class Material(C.Structure):
     _fields_ = [
         ("acoustics", C.c_int32),
@@ -113,6 +121,7 @@ class Material(C.Structure):
         ("zoe_deg", C.c_void_p),
         ("zoe_Rpp", C.c_void_p),
         ("zoe_Rsvsv", C.c_void_p),
+        ("lds", LdsTable * 4),
     ]
 
 
@@ -259,6 +268,9 @@ class MaterialParams(C.Structure):
         ("knot_threshold", C.c_double),
         ("n_theta", C.c_int64),
         ("rho", C.c_double),
+        ("lds_tol", C.c_double),
+        ("lds_max_knots", C.c_int32),
+        ("pad2_", C.c_int32),
     ]
 
 
@@ -310,6 +322,8 @@ EXPORTS = (
     "rtmc_prepare_materials",
     "rtmc_lc_derivatives",
     "rtmc_zoeppritz_air",
+    "rtmc_compress_table",
+    "rtmc_attach_lds_tables",
     "rtmc_probe_atomics",
     "rtmc_set_trace_timing",
     "rtmc_trace_times",
@@ -364,6 +378,11 @@ def _declare(lib):
     lib.rtmc_zoeppritz_air.restype = C.c_int
     lib.rtmc_zoeppritz_air.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int64, P(C.c_double),
                                        P(C.c_double), P(C.c_double)]
+    lib.rtmc_compress_table.restype = C.c_int
+    lib.rtmc_compress_table.argtypes = [P(C.c_double), P(C.c_double), C.c_int64, C.c_double, C.c_int32, P(LdsKnot),
+                                        P(C.c_int32), P(C.c_double)]
+    lib.rtmc_attach_lds_tables.restype = C.c_int
+    lib.rtmc_attach_lds_tables.argtypes = [P(Material), C.c_int32, C.c_double, C.c_int32, P(C.c_void_p), C.c_void_p]
     lib.rtmc_set_trace_timing.restype = C.c_int
     lib.rtmc_set_trace_timing.argtypes = [C.c_int32]
     lib.rtmc_trace_times.restype = C.c_int

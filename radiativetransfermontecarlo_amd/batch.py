@@ -107,7 +107,7 @@ class Batch:
 
     def __init__(self, geometry, O: Observation, materials: Sequence[MaterialClass], sources, Ns: Sequence[int],
                  seeds: Sequence[int], *, device=None, time_steps: int = 0, precision: str = "fp64",
-                 offsets: Optional[Sequence[int]] = None):
+                 offsets: Optional[Sequence[int]] = None, lds_tol: float = 0.0, lds_max_knots: int = 1024):
         n = len(materials)
         if not (len(Ns) == n and len(seeds) == n):
             raise ValueError("materials, Ns and seeds must have the same length")
@@ -124,7 +124,9 @@ class Batch:
         self.bins = DeviceBins(O, self.dev)
         # prepared MaterialClass objects are uploaded here; anything else carries
         # its rtmc_material in `.c` (native_material.NativeMaterials.items)
-        self.dmats = [DeviceMaterial(m, self.d, self.dev, with_zoeppritz=self.geometry.n_bnd > 0)
+        # (lds_tol > 0: with compressed angle tables, which the kernel holds in LDS)
+        self.dmats = [DeviceMaterial(m, self.d, self.dev, with_zoeppritz=self.geometry.n_bnd > 0, lds_tol=lds_tol,
+                                     lds_max_knots=lds_max_knots)
                       if isinstance(m, MaterialClass) else m for m in materials]
         self._tables = None
         self._native = None
@@ -184,13 +186,16 @@ class Batch:
 
     @classmethod
     def from_params(cls, geometry, O: Observation, base: MaterialClass, params, sources, Ns: Sequence[int],
-                    seeds: Sequence[int], *, knot_threshold: float = 0.0, n_glc: int = 2049, **kw) -> "Batch":
+                    seeds: Sequence[int], *, knot_threshold: float = 0.0, n_glc: int = 2049, lds_tol: float = 0.0,
+                    lds_max_knots: int = 1024, **kw) -> "Batch":
         """A batch of ``base`` at the (c, lc, Q) rows of ``params``, its materials
         and the adjoint's derivative tables prepared natively for all items at
-        once (rtmc_prepare_materials, rtmc_lc_derivatives)."""
+        once (rtmc_prepare_materials, rtmc_lc_derivatives); lds_tol > 0 adds the
+        compressed angle tables the kernel holds in LDS."""
         from . import native_material as nm
         d = int(_get(geometry, 'dimension'))
-        plist = [nm.params_for(base, d, c, lc, Q, knot_threshold=knot_threshold) for c, lc, Q in params]
+        plist = [nm.params_for(base, d, c, lc, Q, knot_threshold=knot_threshold, lds_tol=lds_tol,
+                               lds_max_knots=lds_max_knots) for c, lc, Q in params]
         nat = nm.NativeMaterials(plist)
         B = cls(geometry, O, nat.items, sources, Ns, seeds, **kw)
         B._native = nat  # owns the device tables

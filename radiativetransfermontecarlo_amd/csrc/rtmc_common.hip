@@ -79,6 +79,14 @@ int check_material(const rtmc_material* m, const rtmc_geometry* g) {
     int e = check_table(&m->invcdf[k], names[k]);
     if (e) return e;
   }
+  for (int k = 0; k < ntab; ++k) {
+    const rtmc_lds_table& L = m->lds[k];
+    RTMC_REQUIRE(L.m == 0 || (L.m >= 4 && L.m <= 65535 && L.knots != nullptr),
+                 "lds[%d]: a compressed table needs 4..65535 knots and a knot array", k);
+    RTMC_REQUIRE(L.m == 0 || L.extrap == m->lds[0].extrap, "lds[%d]: the tables must share their extrapolation", k);
+    RTMC_REQUIRE((L.m > 0) == (m->lds[0].m > 0), "lds[%d]: either every table of the material is compressed or none",
+                 k);
+  }
   RTMC_REQUIRE(m->vp > 0, "material velocity vp must be > 0");
   RTMC_REQUIRE(m->acoustics || m->vs > 0, "material velocity vs must be > 0 for elastic media");
   RTMC_REQUIRE(m->mft[0] > 0 && (m->acoustics || m->mft[1] > 0), "meanFreeTime must be > 0");
@@ -120,6 +128,12 @@ MatDev to_matdev(const rtmc_material* m) {
   M.zRpp = m->zoe_Rpp;
   M.zRss = m->zoe_Rsvsv;
   M.nz = m->n_zoe;
+  for (int k = 0; k < 4; ++k) {
+    int src = m->acoustics ? 0 : k;
+    M.lk[k] = m->lds[src].knots;
+    M.lm[k] = m->lds[src].knots ? m->lds[src].m : 0;
+  }
+  M.lnear = m->lds[0].extrap == RTMC_EXTRAP_NEAREST;
   return M;
 }
 

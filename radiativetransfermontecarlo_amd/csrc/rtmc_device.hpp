@@ -218,6 +218,10 @@ struct MatDev {
   const double* zRpp;
   const double* zRss;
   int64_t nz;
+  // compressed tables for the tracing kernel's LDS (rtmc_material.lds; lm = 0: none)
+  const rtmc_lds_knot* lk[4];
+  int lm[4];
+  int lnear;  // their extrapolation is RTMC_EXTRAP_NEAREST (the lookup clamps to the end knots)
 };
 
 // ------------------------------------------------------------------

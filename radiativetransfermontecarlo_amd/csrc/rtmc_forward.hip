@@ -263,9 +263,11 @@ int make_prob(const rtmc_material& m, const rtmc_source& s, int64_t n, uint64_t 
 // Shared setup: edges, LDS budget, histogram placement, grid.
 // ev_per_obs: events (scatterings and removals) per observation interval of a
 // particle, weighted over the launch's particles; it sets the default event batch.
+// lt_cap: knots of the largest compressed inverse-CDF table of the launch (0: the
+// problems draw from their full tables in global memory).
 int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt, int time_steps, int precision,
               int acoustics, int block_size, int ppt, int hist_mode, int event_batch, int64_t n_total, int n_prob,
-              bool adjoint, double ev_per_obs, Plan* out) {
+              bool adjoint, double ev_per_obs, int lt_cap, Plan* out) {
   Plan& pl = *out;
   LaunchParams& p = pl.p;
   memset((void*)&p, 0, sizeof(p));
@@ -316,11 +318,19 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   RTMC_REQUIRE(guide >= 0 && guide <= (1 << 16) && (guide & (guide - 1)) == 0,
                "RTMC_GUIDE_CELLS must be 0 or a power of two <= 65536");
   auto guide_bytes = [&](int G) { return G > 0 ? (size_t)p.ntab * (size_t)(G + 1) * sizeof(int) : (size_t)0; };
+  // compressed tables in LDS: per table lt_cap 16-byte knots and a guide of
+  // G + 1 uint16 (G the power of two >= lt_cap, in [64, 4096]), padded to 16 bytes
+  p.lt_cap = lt_cap;
+  p.lt_glog2 = 6;
+  while (p.lt_glog2 < 12 && (1 << p.lt_glog2) < lt_cap) ++p.lt_glog2;
+  p.lt_gstride = (((1 << p.lt_glog2) + 1) + 7) & ~7;
+  const size_t lt_bytes =
+      lt_cap > 0 ? (size_t)p.ntab * ((size_t)lt_cap * sizeof(rtmc_lds_knot) + (size_t)p.lt_gstride * 2) : 0;
   RTMC_REQUIRE(hist_mode >= 0 && hist_mode <= 3, "hist_mode must be 0, 1, 2 or 3");
   bool ldsh;
   if (hist_mode == 1 || hist_mode == 3) ldsh = false;
   else if (hist_mode == 2) ldsh = true;
-  else ldsh = (edge_bytes + hist_bytes) <= (size_t)max_smem && hsize <= (int64_t)1 << 20;
+  else ldsh = (edge_bytes + hist_bytes + lt_bytes) <= (size_t)max_smem && hsize <= (int64_t)1 << 20;
   pl.n_modes = acoustics ? 1 : 2;
   // time-major scratch: explicit (3), or by default whenever the histogram is in global memory
   bool tmaj = !adjoint && !ldsh && n_times >= 2 && (hist_mode == 3 || hist_mode == 0);
@@ -359,16 +369,22 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   }
   pl.hm = ldsh ? HM_LDS : (tmaj ? HM_TMAJOR : HM_GLOBAL);
   pl.tmaj_auto = tmaj && hist_mode == 0;
-  if (ldsh && edge_bytes + hist_bytes > (size_t)max_smem) {
-    set_error("LDS %s of %zu bytes (with %zu bytes of bin edges) exceeds the %d bytes of LDS a work-group has "
-              "beside the kernel's own", adjoint ? "weight table" : "histogram", hist_bytes, edge_bytes, max_smem);
+  if (ldsh && edge_bytes + hist_bytes + lt_bytes > (size_t)max_smem) {
+    set_error("LDS %s of %zu bytes (with %zu bytes of bin edges and %zu of compressed angle tables) exceeds the %d "
+              "bytes of LDS a work-group has beside the kernel's own", adjoint ? "weight table" : "histogram",
+              hist_bytes, edge_bytes, lt_bytes, max_smem);
     return RTMC_ERR_ARG;
   }
+  RTMC_REQUIRE(edge_bytes + lt_bytes <= (size_t)max_smem,
+               "compressed angle tables of %zu bytes (with %zu bytes of bin edges) exceed the %d bytes of LDS a "
+               "work-group has beside the kernel's own: compress them with fewer knots",
+               lt_bytes, edge_bytes, max_smem);
   // The global guide (run_plan) replaces the LDS guide, which then takes no LDS; if
   // the global guide cannot be allocated the lookups search the whole table.
   // RTMC_GLOBAL_GUIDE=0 keeps the LDS guide, which takes what the histogram leaves.
-  pl.global_guide = getenv_int("RTMC_GLOBAL_GUIDE", 1) != 0;
-  if (pl.global_guide) guide = 0;
+  // Compressed tables in LDS need neither.
+  pl.global_guide = getenv_int("RTMC_GLOBAL_GUIDE", 1) != 0 && lt_cap == 0;
+  if (pl.global_guide || lt_cap > 0) guide = 0;
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
   // time-major mode: a cache of hot counters, 8 B a slot: 4096 slots (32 KB), or
@@ -385,10 +401,17 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
                  "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]");
     const size_t budget = 40 * 1024;
     while (guide > 256 && edge_bytes + guide_bytes(guide) + 8 + 8 * (size_t)hot > budget) guide >>= 1;
+    // The compressed tables do not shrink the cache: their instances hold more
+    // registers (3 waves per SIMD for FP64 flight), which leaves them the LDS of 3
+    // work-groups per CU (1e9-particle case, 450-knot table: 1639 ms with 4096 slots,
+    // 1744 ms with 2048).
   }
   p.guide_n = guide;
   p.hist_size = ldsh ? hsize : 0;
   pl.smem = edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0);
+  p.lt_off = (int)((pl.smem + 15) & ~(size_t)15);
+  p.lt_goff = p.lt_off + (int)((size_t)p.ntab * (size_t)lt_cap * sizeof(rtmc_lds_knot));
+  if (lt_cap > 0) pl.smem = (size_t)p.lt_off + lt_bytes;
   p.hot_slots = hot;
   p.hot_times = 1 << 30;
   if (const char* h = getenv("RTMC_HOT_TIMES")) p.hot_times = atoi(h);
@@ -577,7 +600,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Ar
     for (int t = 0; t < pl.p.ntab; ++t) nmax = std::max<int64_t>(nmax, probs[i].M.inv[t].n);
   int lg = 10;
   while (lg < 22 && ((int64_t)1 << lg) < nmax) ++lg;
-  const bool want_gg = pl.global_guide && nmax <= ((int64_t)1 << 31) - 2;
+  const bool want_gg = pl.global_guide && pl.p.lt_cap == 0 && nmax <= ((int64_t)1 << 31) - 2;
   const size_t gcells = ((size_t)1 << lg) + 1;
   const size_t gbytes = sizeof(int) * gcells * (size_t)pl.p.ntab * (size_t)np;
   pl.p.gg_log2 = 0;
@@ -779,9 +802,21 @@ int prepare_batch(const rtmc_batch* bt, const rtmc_adjoint_item* adj, Plan* pl, 
                    (std::max(P.lamS[0], P.lamS[1]) + std::max(P.lamK[0], P.lamK[1]));
   }
   if (probs->empty()) return RTMC_OK;
+  // compressed tables in LDS when every problem has them (RTMC_LDS_TABLE=0: never);
+  // the forward and its adjoint see the same materials, so they draw the same angles
+  int lt_cap = 0;
+  if (getenv_int("RTMC_LDS_TABLE", 1) != 0 && bt->bins.direction_variable == RTMC_DIRVAR_DIRZ) {
+    bool all = true;
+    for (const auto& P : *probs)
+      for (int t = 0; t < (ac ? 1 : 4); ++t) {
+        all = all && P.M.lm[t] > 0;
+        lt_cap = std::max(lt_cap, P.M.lm[t]);
+      }
+    if (!all) lt_cap = 0;
+  }
   e = make_plan(bt->geometry, bt->bins, bt->n_times, bt->dt, bt->time_steps, bt->precision, ac, bt->block_size,
                 bt->particles_per_thread, bt->hist_mode, bt->event_batch, n_total, (int)probs->size(), adj != nullptr,
-                ev_weighted / (double)n_total, pl);
+                ev_weighted / (double)n_total, lt_cap, pl);
   if (e) return e;
   if (adj) {
     for (auto& P : *probs) P.dhist_stride = pl->hsize;

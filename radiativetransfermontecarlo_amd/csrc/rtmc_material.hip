@@ -499,6 +499,96 @@ int parallel_items(int n, F f) {
   return RTMC_OK;
 }
 
+// Knots of the table kept at tolerance tol: 0, 1, n-2, n-1 and, in between,
+// the end of each longest run whose chord from the run's first knot passes
+// within tol of every knot it skips.  The admissible chord slopes from knot s
+// form an interval that each skipped knot narrows (a cone); the run ends
+// where the next knot's own slope leaves it.
+void select_knots(const double* x, const double* y, int64_t n, double tol, std::vector<int64_t>* idx) {
+  idx->clear();
+  if (n <= 4) {
+    for (int64_t i = 0; i < n; ++i) idx->push_back(i);
+    return;
+  }
+  idx->push_back(0);
+  idx->push_back(1);
+  int64_t s = 1, k = 2;
+  double lo = -INFINITY, hi = INFINITY;
+  while (k <= n - 2) {
+    const double dx = x[k] - x[s];
+    const double sl = (y[k] - y[s]) / dx;
+    if (sl >= lo && sl <= hi) {
+      lo = std::max(lo, (y[k] - tol - y[s]) / dx);
+      hi = std::min(hi, (y[k] + tol - y[s]) / dx);
+      ++k;
+    } else {
+      s = k - 1;  // > the previous s: knot s + 1 always fits
+      idx->push_back(s);
+      lo = -INFINITY;
+      hi = INFINITY;
+      k = s + 1;
+    }
+  }
+  if (idx->back() != n - 2) idx->push_back(n - 2);
+  idx->push_back(n - 1);
+}
+
+// rtmc_compress_table on host arrays (checked by the caller)
+void compress_table(const double* x, const double* y, int64_t n, double tol, int max_knots,
+                    std::vector<rtmc_lds_knot>* out, double* tol_used) {
+  std::vector<int64_t> idx;
+  select_knots(x, y, n, tol, &idx);
+  while ((int64_t)idx.size() > max_knots) {
+    tol = tol > 0.0 ? 2.0 * tol : 1e-9;
+    select_knots(x, y, n, tol, &idx);
+  }
+  const size_t m = idx.size();
+  out->resize(m);
+  for (size_t j = 0; j < m; ++j) {
+    const int64_t i = idx[j];
+    rtmc_lds_knot& r = (*out)[j];
+    r.u = x[i];
+    r.theta = (float)y[i];
+    r.slope = 0.0f;
+    if (j + 1 < m) {
+      const int64_t i2 = idx[j + 1];
+      r.slope = (float)((y[i2] - y[i]) / (x[i2] - x[i]));
+    }
+  }
+  *tol_used = tol;
+}
+
+int check_compress_args(const double* x, const double* y, int64_t n, double tol, int32_t max_knots) {
+  RTMC_REQUIRE(x != nullptr && y != nullptr, "table arrays must be given");
+  RTMC_REQUIRE(n >= 2, "the table needs >= 2 knots");
+  RTMC_REQUIRE(tol >= 0.0 && std::isfinite(tol), "tol must be >= 0 and finite");
+  RTMC_REQUIRE(max_knots >= 4 && max_knots <= 65535, "max_knots must be in [4, 65535]");
+  for (int64_t i = 0; i < n; ++i) {
+    RTMC_REQUIRE(std::isfinite(x[i]) && std::isfinite(y[i]), "table knot %lld is not finite", (long long)i);
+    RTMC_REQUIRE(i == 0 || x[i] > x[i - 1], "table knots must be strictly increasing (knot %lld)", (long long)i);
+  }
+  return RTMC_OK;
+}
+
+// The compressed tables of one material from its host tables, appended to `blob`
+// (knot records); lds[c].knots holds the record offset until the upload.
+int compress_material(const double* const* xs, const double* const* ys, const int64_t* ns, int nch, int extrap,
+                      double tol, int max_knots, rtmc_lds_table* lds, std::vector<rtmc_lds_knot>* blob) {
+  for (int c = 0; c < nch; ++c) {
+    int e = check_compress_args(xs[c], ys[c], ns[c], tol, max_knots);
+    if (e) return e;
+    std::vector<rtmc_lds_knot> k;
+    double used = tol;
+    compress_table(xs[c], ys[c], ns[c], tol, max_knots, &k, &used);
+    lds[c].m = (int32_t)k.size();
+    lds[c].extrap = extrap;
+    lds[c].tol = used;
+    lds[c].knots = (const rtmc_lds_knot*)(uintptr_t)blob->size();
+    blob->insert(blob->end(), k.begin(), k.end());
+  }
+  return RTMC_OK;
+}
+
 }  // namespace
 }  // namespace rtmc
 
@@ -555,6 +645,24 @@ extern "C" int rtmc_prepare_material(const rtmc_material_params* p, rtmc_materia
     out->invcdf[c].y = (const double*)dy;
     out->invcdf[c].extrap = h.extrap;
   }
+  if (rc == RTMC_OK && p->lds_tol > 0.0) {
+    // compressed copies of the tables for the tracing kernel's LDS
+    std::vector<rtmc_lds_knot> blob;
+    const double* xs[4] = {h.x[0], h.x[1], h.x[2], h.x[3]};
+    const double* ys[4] = {h.y[0], h.y[1], h.y[2], h.y[3]};
+    rc = compress_material(xs, ys, h.n_knots, nch, h.extrap, p->lds_tol,
+                           p->lds_max_knots > 0 ? p->lds_max_knots : 1024, out->lds, &blob);
+    void* dk = nullptr;
+    if (rc == RTMC_OK && (hipMalloc(&dk, blob.size() * sizeof(rtmc_lds_knot)) != hipSuccess ||
+                          hipMemcpy(dk, blob.data(), blob.size() * sizeof(rtmc_lds_knot), hipMemcpyHostToDevice) !=
+                              hipSuccess)) {
+      set_error("uploading the compressed inverse-CDF tables failed");
+      rc = RTMC_ERR_NOMEM;
+    }
+    if (dk) dt->ptrs.push_back(dk);
+    for (int c = 0; c < nch && rc == RTMC_OK; ++c)
+      out->lds[c].knots = (const rtmc_lds_knot*)dk + (uintptr_t)out->lds[c].knots;
+  }
   rtmc_free_material_host(&h);
   if (rc == RTMC_OK && !p->acoustics && p->rho > 0.0) {
     // solid/air energy coefficients for elastic reflective boundaries
@@ -594,18 +702,35 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
   auto free_all = [&]() {
     for (auto& x : h) rtmc_free_material_host(&x);
   };
-  int e = parallel_items(n, [&](int i) { return prepare_host(&params[i], &h[(size_t)i], 1); });
+  // compressed LDS copies of the tables (lds_tol > 0), per material
+  std::vector<std::vector<rtmc_lds_knot>> lblob((size_t)n);
+  std::vector<rtmc_lds_table> ltab((size_t)n * 4);
+  int e = parallel_items(n, [&](int i) {
+    const rtmc_material_params* p = &params[i];
+    int r = prepare_host(p, &h[(size_t)i], 1);
+    if (r || !(p->lds_tol > 0.0)) return r;
+    const rtmc_material_host& hi = h[(size_t)i];
+    const double* xs[4] = {hi.x[0], hi.x[1], hi.x[2], hi.x[3]};
+    const double* ys[4] = {hi.y[0], hi.y[1], hi.y[2], hi.y[3]};
+    return compress_material(xs, ys, hi.n_knots, p->acoustics ? 1 : 4, hi.extrap, p->lds_tol,
+                             p->lds_max_knots > 0 ? p->lds_max_knots : 1024, &ltab[(size_t)i * 4],
+                             &lblob[(size_t)i]);
+  });
   if (e) {
     free_all();
     return e;
   }
-  // one device block for every table: x then y of each channel, 8-byte aligned
+  // one device block for every table: x then y of each channel, 8-byte aligned,
+  // then the compressed knots (16-byte records, 16-byte aligned)
   size_t total = 0;
   for (int i = 0; i < n; ++i) {
     const int nch = params[i].acoustics ? 1 : 4;
     for (int c = 0; c < nch; ++c) total += 2 * (size_t)h[(size_t)i].n_knots[c];
     if (!params[i].acoustics && params[i].rho > 0.0) total += 3 * (size_t)kZoeAngles;
   }
+  total += total & 1;
+  const size_t lds_off = total;  // in doubles
+  for (int i = 0; i < n; ++i) total += 2 * lblob[(size_t)i].size();
   void* dev = nullptr;
   if (hipMalloc(&dev, total * sizeof(double)) != hipSuccess) {
     free_all();
@@ -650,6 +775,18 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
       m.zoe_Rsvsv = (const double*)dev + off + 2 * kZoeAngles;
       off += 3 * (size_t)kZoeAngles;
     }
+  }
+  for (int i = 0, loff = 0; i < n; ++i) {
+    const std::vector<rtmc_lds_knot>& b = lblob[(size_t)i];
+    if (b.empty()) continue;
+    rtmc_lds_knot* base = (rtmc_lds_knot*)((double*)dev + lds_off) + loff;
+    copied = copied && hipMemcpy(base, b.data(), b.size() * sizeof(rtmc_lds_knot), hipMemcpyHostToDevice) == hipSuccess;
+    for (int c = 0; c < (params[i].acoustics ? 1 : 4); ++c) {
+      rtmc_lds_table t = ltab[(size_t)i * 4 + c];
+      t.knots = base + (uintptr_t)t.knots;
+      out[i].lds[c] = t;
+    }
+    loff += (int)b.size();
   }
   free_all();
   if (!copied) {
@@ -712,6 +849,92 @@ extern "C" int rtmc_zoeppritz_air(double vp, double vs, double rho, int64_t n, d
   RTMC_REQUIRE(vp > 0.0 && vs > 0.0 && rho > 0.0 && std::isfinite(vp) && std::isfinite(vs) && std::isfinite(rho),
                "vp, vs and rho must be positive and finite");
   zoeppritz_air(vp, vs, rho, n, deg, E_Rpp, E_Rsvsv);
+  return RTMC_OK;
+}
+
+extern "C" int rtmc_compress_table(const double* x, const double* y, int64_t n, double tol, int32_t max_knots,
+                                   rtmc_lds_knot* out, int32_t* m, double* tol_used) {
+  clear_error();
+  RTMC_REQUIRE(out != nullptr && m != nullptr && tol_used != nullptr, "outputs must be given");
+  int e = check_compress_args(x, y, n, tol, max_knots);
+  if (e) return e;
+  std::vector<rtmc_lds_knot> k;
+  compress_table(x, y, n, tol, max_knots, &k, tol_used);
+  memcpy(out, k.data(), k.size() * sizeof(rtmc_lds_knot));
+  *m = (int32_t)k.size();
+  return RTMC_OK;
+}
+
+extern "C" int rtmc_attach_lds_tables(rtmc_material* mats, int32_t n, double tol, int32_t max_knots, void** handle,
+                                      void* stream) {
+  clear_error();
+  RTMC_REQUIRE(handle != nullptr, "handle must be given");
+  *handle = nullptr;
+  RTMC_REQUIRE(n >= 0 && (n == 0 || mats != nullptr), "materials must be given");
+  RTMC_REQUIRE(tol >= 0.0 && std::isfinite(tol), "tol must be >= 0 and finite");
+  if (max_knots == 0) max_knots = 1024;
+  RTMC_REQUIRE(max_knots >= 4 && max_knots <= 65535, "max_knots must be in [4, 65535]");
+  if (n == 0) return RTMC_OK;
+  for (int i = 0; i < n; ++i) {
+    const int nch = mats[i].acoustics ? 1 : 4;
+    for (int c = 0; c < nch; ++c) {
+      const rtmc_table& t = mats[i].invcdf[c];
+      RTMC_REQUIRE(t.n >= 2 && t.x != nullptr && t.y != nullptr, "material %d: inverse-CDF table %d is missing", i, c);
+    }
+  }
+  // the tables may still be written by work queued on `stream`
+  RTMC_HIP(hipStreamSynchronize((hipStream_t)stream));
+  std::vector<std::vector<rtmc_lds_knot>> blob((size_t)n);
+  std::vector<rtmc_lds_table> tab((size_t)n * 4);
+  int e = parallel_items(n, [&](int i) -> int {
+    const rtmc_material& M = mats[i];
+    const int nch = M.acoustics ? 1 : 4;
+    std::vector<double> hx[4], hy[4];
+    const double* xs[4] = {nullptr, nullptr, nullptr, nullptr};
+    const double* ys[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t ns[4] = {0, 0, 0, 0};
+    for (int c = 0; c < nch; ++c) {
+      const rtmc_table& t = M.invcdf[c];
+      hx[c].resize((size_t)t.n);
+      hy[c].resize((size_t)t.n);
+      RTMC_HIP(hipMemcpy(hx[c].data(), t.x, sizeof(double) * (size_t)t.n, hipMemcpyDeviceToHost));
+      RTMC_HIP(hipMemcpy(hy[c].data(), t.y, sizeof(double) * (size_t)t.n, hipMemcpyDeviceToHost));
+      xs[c] = hx[c].data();
+      ys[c] = hy[c].data();
+      ns[c] = t.n;
+      RTMC_REQUIRE(t.extrap == M.invcdf[0].extrap, "the tables of one material must share their extrapolation");
+    }
+    return compress_material(xs, ys, ns, nch, M.invcdf[0].extrap, tol, max_knots, &tab[(size_t)i * 4],
+                             &blob[(size_t)i]);
+  });
+  if (e) return e;
+  size_t total = 0;
+  for (auto& b : blob) total += b.size();
+  void* dev = nullptr;
+  if (hipMalloc(&dev, total * sizeof(rtmc_lds_knot)) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipMalloc of %zu bytes of compressed tables failed", total * sizeof(rtmc_lds_knot));
+    return RTMC_ERR_NOMEM;
+  }
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    const std::vector<rtmc_lds_knot>& b = blob[(size_t)i];
+    if (hipMemcpy((rtmc_lds_knot*)dev + off, b.data(), b.size() * sizeof(rtmc_lds_knot), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      (void)hipFree(dev);
+      set_error("uploading the compressed tables failed");
+      return RTMC_ERR_HIP;
+    }
+    for (int c = 0; c < (mats[i].acoustics ? 1 : 4); ++c) {
+      rtmc_lds_table t = tab[(size_t)i * 4 + c];
+      t.knots = (const rtmc_lds_knot*)dev + off + (uintptr_t)t.knots;
+      mats[i].lds[c] = t;
+    }
+    off += b.size();
+  }
+  DeviceTables* dt = new DeviceTables();
+  dt->ptrs.push_back(dev);
+  *handle = dt;
   return RTMC_OK;
 }
 

@@ -163,6 +163,15 @@ struct LaunchParams {
   int tw;             // runs of three: words per row and shift, ceil(n_times / 3)
   int run_n;          // runs of three: particles per scratch copy (<= 2^21 - 1)
   int64_t run_words;  // runs of three: 64-bit words of one scratch copy
+  // compressed inverse-CDF tables held in LDS (lt_cap > 0; MatDev::lk): table t's
+  // knot records at byte lt_off + 16 t lt_cap, its guide (2^lt_glog2 + 1 uint16:
+  // cell c holds the largest knot k <= m-2 with u_k <= c / 2^lt_glog2, 0 below
+  // u_0) at byte lt_goff + 2 t lt_gstride
+  int lt_cap;
+  int lt_glog2;
+  int lt_gstride;
+  int lt_off;
+  int lt_goff;
   int n_prob;
   unsigned long long* work;  // per-problem particle counters (zeroed)
   unsigned int* active;      // per-problem count of work-groups on it (zeroed)
@@ -604,7 +613,9 @@ constexpr int kTraceStaticLds = 256;
 static_assert(sizeof(BinSetup) + 8 * sizeof(double) + 2 * sizeof(int) + 32 <= (size_t)kTraceStaticLds,
               "kTraceStaticLds must cover the kernel's static LDS");
 
-template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK, bool POS>
+// LT: scattering angles from the compressed tables held in LDS (LaunchParams::lt_cap
+// > 0; its own instances, so the full-table instances keep their registers)
+template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK, bool POS, bool LT>
 __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const LaunchParams p,
                                                                      const ProbDev* __restrict__ probs) {
   constexpr bool LDSH = HM == HM_LDS;
@@ -625,6 +636,8 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
   // and the host keeps a work-group's particles below 2^24)
   constexpr unsigned long long kEmpty = ~0ull;
   unsigned long long* hslot = (unsigned long long*)((char*)smem + p.hot_off);
+  rtmc_lds_knot* sknots = (rtmc_lds_knot*)((char*)smem + p.lt_off);
+  uint16_t* sltg = (uint16_t*)((char*)smem + p.lt_goff);
 
   if (p.edges_total > 0) {
     // gather the edge arrays into one LDS block
@@ -686,6 +699,43 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
         }
       }
     }
+    if (LT) {
+      // the problem's compressed tables into LDS, then their guides; the clamp
+      // bounds s_tx0 / s_tx1 hold NEAREST lookups to the end knots
+      for (int t = 0; t < p.ntab; ++t) {
+        const int m = M.lm[t];
+        const RTMC_GLOBAL rtmc_lds_knot* src = gl(M.lk[t]);
+        rtmc_lds_knot* dst = sknots + t * p.lt_cap;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+          dst[k].u = src[k].u;
+          dst[k].theta = src[k].theta;
+          dst[k].slope = src[k].slope;
+        }
+        if (threadIdx.x == 0) {
+          s_tx0[t] = M.lnear ? src[0].u : -INFINITY;
+          s_tx1[t] = M.lnear ? src[m - 1].u : INFINITY;
+        }
+      }
+      __syncthreads();
+      const int G = 1 << p.lt_glog2;
+      const double invG = 1.0 / (double)G;  // exact
+      for (int t = 0; t < p.ntab; ++t) {
+        const int m = M.lm[t];
+        const rtmc_lds_knot* kn = sknots + t * p.lt_cap;
+        for (int c = threadIdx.x; c <= G; c += blockDim.x) {
+          const double q = (double)c * invG;
+          int lo = 0, hi = m - 1;
+          if (q >= kn[0].u) {
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (kn[mid].u <= q) lo = mid;
+              else hi = mid;
+            }
+          }
+          sltg[t * p.lt_gstride + c] = (uint16_t)lo;
+        }
+      }
+    }
     if (LDSH) {
       if (ADJ) {
         for (int64_t k = threadIdx.x; k < p.hist_size; k += blockDim.x) sw[k] = P.w[k];
@@ -701,6 +751,23 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     // inverse-CDF lookup of the reference table (same interval and formula as
     // table_eval), bracketed by the LDS guide when present
     auto lookup = [&](int t, double u) -> double {
+      if (LT) {
+        // compressed table in LDS: the guide brackets the knot, a short search in
+        // LDS finds it, and the chord from it gives theta
+        const double q = fmin(fmax(u, s_tx0[t]), s_tx1[t]);
+        const int G = 1 << p.lt_glog2;
+        int c = (int)(q * (double)G);
+        c = c < 0 ? 0 : (c > G - 1 ? G - 1 : c);
+        const uint16_t* g = sltg + t * p.lt_gstride;
+        int lo = g[c], hi = (int)g[c + 1] + 1;
+        const rtmc_lds_knot* kn = sknots + t * p.lt_cap;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (kn[mid].u <= q) lo = mid;
+          else hi = mid;
+        }
+        return (double)kn[lo].theta + (q - kn[lo].u) * (double)kn[lo].slope;
+      }
       const TableDev& T = M.inv[t];
       const RTMC_GLOBAL double* const Tx = gl(T.x);
       const RTMC_GLOBAL double* const Ty = gl(T.y);

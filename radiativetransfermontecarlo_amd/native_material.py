@@ -18,11 +18,17 @@ from . import _abi
 from .material import MaterialClass
 
 __all__ = ["params_from", "params_for", "prepare_host", "NativeMaterial", "NativeMaterials", "lc_derivatives",
-           "zoeppritz_air"]
+           "zoeppritz_air", "compress_table", "KNOT_DTYPE"]
+
+# rtmc_lds_knot as a numpy record
+KNOT_DTYPE = np.dtype([("u", np.float64), ("theta", np.float32), ("slope", np.float32)])
 
 
-def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n_theta: int = 0):
-    """rtmc_material_params of an (unprepared or prepared) isotropic MaterialClass."""
+def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n_theta: int = 0,
+                lds_tol: float = 0.0, lds_max_knots: int = 0):
+    """rtmc_material_params of an (unprepared or prepared) isotropic MaterialClass.
+    lds_tol > 0 also asks for compressed copies of the angle tables for the
+    tracing kernel's LDS (rtmc_compress_table)."""
     if material.type != 'isotropic':
         raise ValueError("native preparation covers isotropic media (calcSigmaIsotropic)")
     law = material.SpectralLaw
@@ -59,7 +65,25 @@ def params_from(material: MaterialClass, d: int, knot_threshold: float = 1e-8, n
     # an elastic density gives the solid/air tables of reflective boundaries
     rho = getattr(material, 'rho', None)
     p.rho = float(rho) if (rho is not None and not material.acoustics) else 0.0
+    p.lds_tol = float(lds_tol)
+    p.lds_max_knots = int(lds_max_knots)
     return p
+
+
+def compress_table(x, y, tol: float = 1e-5, max_knots: int = 1024):
+    """rtmc_compress_table on host arrays: (knots as a KNOT_DTYPE record array,
+    the tolerance achieved)."""
+    lib = _abi.load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(int(max_knots), dtype=KNOT_DTYPE)
+    m = C.c_int32()
+    used = C.c_double()
+    P = C.POINTER(C.c_double)
+    _abi.check(lib.rtmc_compress_table(x.ctypes.data_as(P), y.ctypes.data_as(P), x.size, float(tol), int(max_knots),
+                                       out.ctypes.data_as(C.POINTER(_abi.LdsKnot)), C.byref(m), C.byref(used)),
+               "rtmc_compress_table")
+    return out[:m.value].copy(), used.value
 
 
 def zoeppritz_air(vp: float, vs: float, rho: float, n: int = 181) -> dict:
@@ -74,10 +98,11 @@ def zoeppritz_air(vp: float, vs: float, rho: float, n: int = 181) -> dict:
 
 
 def params_for(base: MaterialClass, d: int, c: float, lc: float, Q: float, knot_threshold: float = 1e-8,
-               n_theta: int = 0):
+               n_theta: int = 0, lds_tol: float = 0.0, lds_max_knots: int = 0):
     """params_from of ``base`` with the coefficients of variation scaled to
     cv[0] = c, the correlation length lc and the attenuation Q (acoustic)."""
-    p = params_from(base, d, knot_threshold=knot_threshold, n_theta=n_theta)
+    p = params_from(base, d, knot_threshold=knot_threshold, n_theta=n_theta, lds_tol=lds_tol,
+                    lds_max_knots=lds_max_knots)
     s = float(c) / p.cv[0]
     for k in range(3):
         p.cv[k] *= s

@@ -186,9 +186,13 @@ def _radial_inverse_cdf(f, d: int) -> InverseCDF:
 # material on the device
 
 class DeviceMaterial:
-    """A prepared MaterialClass with its tables uploaded (rtmc_material)."""
+    """A prepared MaterialClass with its tables uploaded (rtmc_material).
+    lds_tol > 0 also uploads compressed copies of the angle tables
+    (rtmc_compress_table with this tolerance in radians and at most
+    lds_max_knots knots each), which the tracing kernel then holds in LDS."""
 
-    def __init__(self, material: MaterialClass, d: int, device, with_zoeppritz: bool = False):
+    def __init__(self, material: MaterialClass, d: int, device, with_zoeppritz: bool = False,
+                 lds_tol: float = 0.0, lds_max_knots: int = 1024):
         self.keep = []
         m = _abi.Material()
         m.acoustics = 1 if material.acoustics else 0
@@ -227,6 +231,15 @@ class DeviceMaterial:
             m.invcdf[k].x = tx.data_ptr()
             m.invcdf[k].y = ty.data_ptr()
             m.invcdf[k].extrap = ex
+            if lds_tol > 0.0:
+                from .native_material import compress_table
+                kn, used = compress_table(x, y, lds_tol, lds_max_knots)
+                tk = torch.from_numpy(kn.view(np.uint8).copy()).to(device)
+                self.keep.append(tk)
+                m.lds[k].m = kn.size
+                m.lds[k].extrap = ex
+                m.lds[k].tol = used
+                m.lds[k].knots = tk.data_ptr()
         if with_zoeppritz and not material.acoustics:
             out = MaterialClass.Zoeppritz(material)
             deg = _f64(out['j1_deg'], device)

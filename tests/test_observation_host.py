@@ -93,10 +93,11 @@ def test_struct_layouts_match_header():
     src = r'''
 #include <stdio.h>
 #include "rtmc.h"
-int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rtmc_geometry),
+int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rtmc_geometry),
  sizeof(rtmc_table), sizeof(rtmc_material), sizeof(rtmc_source), sizeof(rtmc_particles), sizeof(rtmc_draws),
  sizeof(rtmc_bins), sizeof(rtmc_problem), sizeof(rtmc_substep_info), sizeof(rtmc_batch_item), sizeof(rtmc_batch),
- sizeof(rtmc_adjoint_item), sizeof(rtmc_tuning), sizeof(rtmc_material_params), sizeof(rtmc_material_host));
+ sizeof(rtmc_adjoint_item), sizeof(rtmc_tuning), sizeof(rtmc_material_params), sizeof(rtmc_material_host),
+ sizeof(rtmc_lds_knot), sizeof(rtmc_lds_table));
  return 0;}
 '''
     with tempfile.TemporaryDirectory() as d:
@@ -108,5 +109,5 @@ int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n
     mine = [ctypes.sizeof(t) for t in (_abi.Geometry, _abi.Table, _abi.Material, _abi.Source, _abi.Particles,
                                        _abi.Draws, _abi.Bins, _abi.Problem, _abi.SubstepInfo, _abi.BatchItem,
                                        _abi.Batch, _abi.AdjointItem, _abi.Tuning, _abi.MaterialParams,
-                                       _abi.MaterialHost)]
+                                       _abi.MaterialHost, _abi.LdsKnot, _abi.LdsTable)]
     assert sizes == mine
