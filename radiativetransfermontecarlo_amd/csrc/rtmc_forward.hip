@@ -342,10 +342,18 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   p.tsp = (uint32_t)(p.tcomp ? B.stride[3] : B.time_stride);
   pl.tsize = tmaj ? (int64_t)p.tsp * pl.n_modes * (int64_t)p.nt_pad : 0;
   // a second scratch shifted by one time takes the pairs that start at odd times
-  // when the scratch is small (its clearing and transpose-add are then cheap)
+  // when the two fit in a quarter of the device's memory (1e9-particle case, two
+  // 28.4 GB scratches: step 2025 -> 1946 ms, with the compressed angle tables
+  // 1660 -> 1604 ms; run_plan drops it when the allocation fails)
   {
-    const int op = getenv_int("RTMC_ODD_PAIRS", 1);  // 0 off, 1 for scratches <= 2 GiB, 2 always
-    p.odd_pairs = tmaj && (op == 2 || (op == 1 && pl.tsize * 4 <= ((int64_t)2 << 30)));
+    const int op = getenv_int("RTMC_ODD_PAIRS", 1);  // 0 off, 1 when they fit, 2 always
+    size_t mem_free = 0, mem_total = 0;
+    if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) {
+      (void)hipGetLastError();
+      mem_total = 0;
+    }
+    const int64_t lim = std::max<int64_t>((int64_t)2 << 30, (int64_t)(mem_total / 8));
+    p.odd_pairs = tmaj && (op == 2 || (op == 1 && pl.tsize * 4 <= lim));
   }
   // runs of three (21-bit counters, one scratch copy per run_particles
   // particles) when all the copies fit in 4 GiB
@@ -633,6 +641,13 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Ar
       }
     }
     hipError_t ae = arena_alloc(&ar, &tbuf, tbytes, st);
+    if (ae != hipSuccess && !pl.p.runs3 && pl.p.odd_pairs) {
+      // without the second scratch of pairs from odd times
+      (void)hipGetLastError();
+      pl.p.odd_pairs = 0;
+      tbytes = sizeof(uint32_t) * tmajor_words(pl, probs, &nsub, &toff);
+      ae = arena_alloc(&ar, &tbuf, tbytes, st);
+    }
     if (ae != hipSuccess) {
       tbuf = nullptr;
       (void)hipGetLastError();
