@@ -16,7 +16,9 @@
  *     N, seeds   one value per material, or one value for all (seeds then seed + i - 1)
  *     options    struct: precision 'fp64' | 'fp32', hist_mode, autotune (fraction of the
  *                particles to tune the launch on; 0 = off), knot_threshold (default 1e-8,
- *                the reference's tables; 0 keeps every knot, which the adjoint assumes)
+ *                the reference's tables; 0 keeps every knot, which the adjoint assumes),
+ *                lds_tol (radians; > 0: scattering angles from compressed tables held in
+ *                LDS, within lds_tol of the full tables; the adjoint replays the same draws)
  *     E          cell array of uint32 [N1 N2 Nt 1+~acoustics], one per material
  *     counters   n x 4 double (scattering events, deposits, boundary interactions, removals)
  *     id         with three outputs, the batch is kept on the device for 'adjoint'
@@ -129,7 +131,7 @@ static void forward(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) 
     char prec[16];
     mwSize n1 = 0, n2 = 0;
     size_t n, total;
-    double knot, tune;
+    double knot, tune, lds_tol;
     int slot = -1;
     if (!b) mexErrMsgIdAndTxt("rtmc:nomem", "out of host memory");
     pending = b;
@@ -145,8 +147,11 @@ static void forward(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) 
     rtmc_mex_geometry(k, geo, &b->batch.geometry);
     rtmc_mex_bins(k, obs, ibins, RTMC_DIRVAR_DIRZ, &b->batch.bins, &b->batch.n_times, &b->batch.dt, &n1, &n2);
     knot = rtmc_mex_scalar(k, opt, "knot_threshold", 1e-8);
-    for (size_t i = 0; i < n; ++i)
+    lds_tol = rtmc_mex_scalar(k, opt, "lds_tol", 0.0);
+    for (size_t i = 0; i < n; ++i) {
         params_from(k, mxGetCell(mats, (mwIndex)i), b->batch.geometry.dimension, knot, &b->params[i]);
+        b->params[i].lds_tol = lds_tol;  /* > 0: compressed angle tables held in LDS */
+    }
     rtmc_mex_check(k, rtmc_prepare_materials(b->params, (int32_t)n, dm, &b->mat_handle), "rtmc_prepare_materials");
 
     b->dims[0] = n1;
@@ -257,6 +262,7 @@ static void adjoint(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) 
     if (!mxIsCell(dl) || mxGetNumberOfElements(dl) != n)
         mexErrMsgIdAndTxt("rtmc:arg", "dLdE must be a cell array with one array per material");
     tmp.n = 0;
+    tmp.nh = 0;
     total = (size_t)b->dims[0] * b->dims[1] * b->dims[2];
     a = (rtmc_adjoint_item*)mxCalloc(n, sizeof(rtmc_adjoint_item));
     g = (double*)mxCalloc(n * N_THETA_DERIV, sizeof(double));

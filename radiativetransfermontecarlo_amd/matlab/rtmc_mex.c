@@ -20,7 +20,9 @@
  *   N         particles;  seed  Philox key (any integer below 2^53)
  *   options   (optional) struct: precision 'fp64' (default) | 'fp32' (FP32 free flight,
  *             FP64 time), hist_mode 0..3 (RTMC_HIST_*), direction_variable 'dirz' (default,
- *             observeTime.m) | 'position' (README.md:209)
+ *             observeTime.m) | 'position' (README.md:209), lds_tol (radians; > 0: draw the
+ *             scattering angles from compressed copies of invcdf held in LDS, within lds_tol
+ *             of the full tables, rtmc_attach_lds_tables; dirz only)
  * Outputs
  *   energy    uint32 [N1 N2 Nt 1+~acoustics], the energy array initializeObservation
  *             allocates, filled
@@ -86,6 +88,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     uint64_t* dC;
     uint64_t hc[RTMC_NCOUNTERS];
     keep.n = 0;
+    keep.nh = 0;
     if (nrhs < 7 || nrhs > 8 || nlhs > 2)
         mexErrMsgIdAndTxt("rtmc:usage", "[energy, counters] = rtmc_mex(geometry, source, material, obs, ibins, N, seed[, options])");
     if (rtmc_abi_version() != RTMC_ABI_VERSION)
@@ -99,6 +102,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     rtmc_mex_string(&keep, opt, "direction_variable", dv, sizeof(dv));
     rtmc_mex_bins(&keep, prhs[3], prhs[4], strcmp(dv, "position") == 0 ? RTMC_DIRVAR_POSITION : RTMC_DIRVAR_DIRZ,
                   &p.bins, &p.n_times, &p.dt, &n1, &n2);
+    {
+        const double lds_tol = rtmc_mex_scalar(&keep, opt, "lds_tol", 0.0);
+        if (lds_tol > 0.0) {
+            void* h = NULL;
+            rtmc_mex_check(&keep, rtmc_attach_lds_tables(&p.material, 1, lds_tol, 0, &h, NULL), "rtmc_attach_lds_tables");
+            keep.h[keep.nh++] = h;
+        }
+    }
 
     p.n_particles = (int64_t)rtmc_mex_num(&keep, prhs[5], 0, "N");
     p.seed = (uint64_t)rtmc_mex_num(&keep, prhs[6], 0, "seed");

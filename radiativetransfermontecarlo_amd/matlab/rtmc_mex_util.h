@@ -20,16 +20,22 @@
 #include "mex.h"
 #include "rtmc.h"
 
-/* Device allocations of one gateway call; freed on every exit path. */
+/* Device allocations of one gateway call (and library handles of material
+ * tables, rtmc_free_material); freed on every exit path. */
 #define RTMC_MEX_MAX_KEEP 256
+#define RTMC_MEX_MAX_HANDLES 8
 typedef struct {
     void* p[RTMC_MEX_MAX_KEEP];
     int n;
+    void* h[RTMC_MEX_MAX_HANDLES];
+    int nh;
 } rtmc_mex_keep;
 
 static void rtmc_mex_release(rtmc_mex_keep* k) {
     for (int i = 0; i < k->n; ++i) (void)hipFree(k->p[i]);
     k->n = 0;
+    for (int i = 0; i < k->nh; ++i) (void)rtmc_free_material(k->h[i]);
+    k->nh = 0;
 }
 
 /* Frees the call's device memory, then raises the MATLAB error (does not return). */
