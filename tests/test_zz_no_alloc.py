@@ -5,8 +5,9 @@ bit-exact, on two calls in a row, the second with a smaller input.
 The cases take each placement the no-alloc path can choose within its
 64 MiB array: the LDS histogram; the time-major scratch with runs of three;
 pairs from even times only (runs of three and both pair scratches too large);
-direct atomics (no scratch fits); and an explicit time-major request that does
-not fit, which must fail with RTMC_ERR_ARG.
+direct atomics (no scratch fits); an explicit time-major request that does
+not fit, which must fail with RTMC_ERR_ARG; and materials with compressed
+angle tables held in LDS (no guide in the array).
 """
 import ctypes as C
 import math
@@ -25,13 +26,14 @@ pytestmark = pytest.mark.gpu
 LIMIT = 64 << 20
 
 
-def _problem(geometry, source, material, O, N, seed, dev, hist_mode=0):
+def _problem(geometry, source, material, O, N, seed, dev, hist_mode=0, lds_tol=0.0):
     d = int(geometry['dimension'])
     keep = []
     prob = _abi.Problem()
     prob.geometry = solver.build_geometry(geometry)
     prob.source = solver.build_source(source, d, keep, dev)
-    dmat = solver.DeviceMaterial(material, d, dev, with_zoeppritz=prob.geometry.n_bnd > 0)
+    dmat = solver.DeviceMaterial(material, d, dev, with_zoeppritz=prob.geometry.n_bnd > 0, lds_tol=lds_tol,
+                                 lds_max_knots=256)
     prob.material = dmat.c
     bins = solver.DeviceBins(O, dev)
     prob.bins = bins.c
@@ -53,12 +55,12 @@ def _run(entry, prob, size, dev):
     return rc, E, cnt
 
 
-def _both(geometry, source, material, observation, N, seed, dev, hist_mode=0):
+def _both(geometry, source, material, observation, N, seed, dev, hist_mode=0, lds_tol=0.0):
     d = geometry['dimension']
     mat = MaterialClass.prepareSigma(material, d)
     O = initializeObservation(geometry, bool(mat.acoustics), observation, N, allow_nd=True)
-    prob, keep = _problem(geometry, source, mat, O, N, seed, dev, hist_mode)
-    print(f"no-alloc case: d={d} N={N} hist={O.size} words, seed={seed}", flush=True)
+    prob, keep = _problem(geometry, source, mat, O, N, seed, dev, hist_mode, lds_tol)
+    print(f"no-alloc case: d={d} N={N} hist={O.size} words, seed={seed}, lds_tol={lds_tol}", flush=True)
     rc_a, Ea, ca = _run("rtmc_forward", prob, O.size, dev)
     _abi.check(rc_a, "rtmc_forward")
     rc_b, Eb, cb = _run("rtmc_forward_noalloc", prob, O.size, dev)
@@ -121,6 +123,19 @@ def test_direct_atomics_when_nothing_fits_then_smaller(gpu):
     _both(g, s, m, o, 60_000, 31, gpu)
     g, s, m, o = _config2_small(20, 2)
     _both(g, s, m, o, 20_000, 32, gpu)
+
+
+def test_compressed_angle_tables_two_calls(gpu):
+    """Angles from compressed tables in LDS (their knots are the material's own
+    device memory; the array holds no guide): the README case, a 2-D elastic
+    grid whose runs-of-three scratch fits, then the README case with fewer
+    particles."""
+    geometry, source, material, observation = cases.config1()
+    _both(geometry, source, material, observation, 200_000, 51, gpu, lds_tol=1e-4)
+    g, s, m, o = _config2_small(10, 4)
+    _both(g, s, m, o, 100_000, 52, gpu, lds_tol=1e-4)
+    observation = dict(observation, time=np.arange(0, 1 + 1e-9, 0.02))
+    _both(geometry, source, material, observation, 20_000, 53, gpu, lds_tol=1e-4)
 
 
 def test_explicit_time_major_too_large_is_rejected(gpu):
