@@ -257,7 +257,8 @@ class EnergyDensity(torch.autograd.Function):
     """energyDensity (n x O.size) of n acoustic materials as a function of
     their (c, lc, Q) (an n x 3 float64 tensor), with the adjoint kernel as
     backward.  ``spec`` = dict(base=MaterialClass, geometry, O, sources, Ns,
-    seeds, device, time_steps, precision)."""
+    seeds, device, time_steps, precision, and optionally knot_threshold,
+    tuning, lds_tol)."""
 
     @staticmethod
     def forward(ctx, params: torch.Tensor, spec: dict):
@@ -267,7 +268,7 @@ class EnergyDensity(torch.autograd.Function):
         mats = [prepare_material(spec["base"], d, c=p[0], lc=p[1], Q=p[2], knot_threshold=kt) for p in P]
         B = Batch(spec["geometry"], spec["O"], mats, spec["sources"], spec["Ns"], spec["seeds"],
                   device=spec.get("device"), time_steps=spec.get("time_steps", 0),
-                  precision=spec.get("precision", "fp64"))
+                  precision=spec.get("precision", "fp64"), lds_tol=spec.get("lds_tol", 0.0))
         if spec.get("tuning"):
             B.configure(**spec["tuning"])
         B.forward()

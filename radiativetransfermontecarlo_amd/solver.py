@@ -454,13 +454,15 @@ def forward_histogram(geometry, source, material: MaterialClass, O: Observation,
 
 def radiativeTransfer(geometry, source, material: MaterialClass, observation, *, seed: Optional[int] = None,
                       engine: str = "fused", device=None, allow_nd: bool = False, return_histogram: bool = False,
-                      precision: str = "fp64"):
+                      precision: str = "fp64", lds_tol: float = 0.0):
     """obs = radiativeTransfer(geometry, source, material, observation)
     (radiativeTransfer.m:1-207) on one GPU.
 
     engine='fused': one persistent kernel for all packets (Philox draws).
     engine='functions': the reference's packet x time loop of function calls.
     precision='fp32' (fused engine): FP32 direction/flight arithmetic, FP64 time.
+    lds_tol > 0 (fused engine): scattering angles from compressed copies of the
+    angle tables held in LDS, within lds_tol radians of the full tables.
     """
     d = int(_get(geometry, 'dimension'))
     if isinstance(geometry, dict) and 'frame' not in geometry:
@@ -485,10 +487,12 @@ def radiativeTransfer(geometry, source, material: MaterialClass, observation, *,
         geometry = dict(geometry)
         geometry.pop('bnd')
     if engine == "fused":
-        dmat = DeviceMaterial(material, d, dev, with_zoeppritz=bool(_get(geometry, 'bnd', None)))
+        dmat = DeviceMaterial(material, d, dev, with_zoeppritz=bool(_get(geometry, 'bnd', None)), lds_tol=lds_tol)
         E, counters, _k = forward_histogram(geometry, source, material, O, N, seed=seed, device=dev,
                                             time_steps=time_steps, dmat=dmat, precision=precision)
     elif engine == "functions":
+        if lds_tol > 0.0:
+            raise ValueError("lds_tol applies to the fused engine (the function-level kernels read the full tables)")
         E = _functions_engine(geometry, source, material, O, N, Np, seed, dev, time_steps)
     else:
         raise ValueError(f"unknown engine {engine!r}")

@@ -196,3 +196,19 @@ def test_lds_table_adjoint_replays_forward(gpu, hist_mode):
     g = B.adjoint(w).cpu().numpy()
     assert torch.equal(B.counters, cnt_f)
     assert np.all(np.isfinite(g))
+
+
+def test_lds_table_dropin_radiative_transfer(gpu):
+    """radiativeTransfer(..., lds_tol=...): the reference's obs fields, every
+    particle counted at every time (Q = Inf, bins covering every reachable
+    radius), and the energy density integrating to the energy."""
+    geometry, source, material, observation = cases.config1(n=150_000)
+    observation = dict(observation, time=np.arange(0, 1 + 1e-9, 0.05))
+    obs = solver.radiativeTransfer(geometry, source, material, observation, seed=3, lds_tol=TOL)
+    assert obs['N'] == 200_000
+    assert obs['energy'][0] == 0.0
+    np.testing.assert_allclose(obs['energy'][1:], 1.0, rtol=0, atol=0)
+    integ = (obs['energyDensity'][:, 0, :] * obs['dx'][:, None]).sum(axis=0) * obs['dy'][0] * obs['dz'][0]
+    np.testing.assert_allclose(integ, obs['energy'], rtol=1e-12)
+    with pytest.raises(ValueError):
+        solver.radiativeTransfer(geometry, source, material, observation, seed=3, lds_tol=TOL, engine="functions")
