@@ -791,6 +791,7 @@ extern "C" int rtmc_prepare_materials(const rtmc_material_params* params, int32_
   free_all();
   if (!copied) {
     (void)hipFree(dev);
+    memset(out, 0, sizeof(rtmc_material) * (size_t)n);  // nothing points at the freed block
     set_error("uploading the material tables failed");
     return RTMC_ERR_NOMEM;
   }
@@ -916,21 +917,27 @@ extern "C" int rtmc_attach_lds_tables(rtmc_material* mats, int32_t n, double tol
     set_error("hipMalloc of %zu bytes of compressed tables failed", total * sizeof(rtmc_lds_knot));
     return RTMC_ERR_NOMEM;
   }
+  // upload everything first: the materials are changed only once nothing can fail
+  std::vector<size_t> base((size_t)n, 0);
   size_t off = 0;
   for (int i = 0; i < n; ++i) {
     const std::vector<rtmc_lds_knot>& b = blob[(size_t)i];
+    base[(size_t)i] = off;
     if (hipMemcpy((rtmc_lds_knot*)dev + off, b.data(), b.size() * sizeof(rtmc_lds_knot), hipMemcpyHostToDevice) !=
         hipSuccess) {
+      (void)hipGetLastError();
       (void)hipFree(dev);
       set_error("uploading the compressed tables failed");
       return RTMC_ERR_HIP;
     }
+    off += b.size();
+  }
+  for (int i = 0; i < n; ++i) {
     for (int c = 0; c < (mats[i].acoustics ? 1 : 4); ++c) {
       rtmc_lds_table t = tab[(size_t)i * 4 + c];
-      t.knots = (const rtmc_lds_knot*)dev + off + (uintptr_t)t.knots;
+      t.knots = (const rtmc_lds_knot*)dev + base[(size_t)i] + (uintptr_t)t.knots;
       mats[i].lds[c] = t;
     }
-    off += b.size();
   }
   DeviceTables* dt = new DeviceTables();
   dt->ptrs.push_back(dev);

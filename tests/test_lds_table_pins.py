@@ -166,3 +166,23 @@ def test_compression_rejects_bad_tables(bad):
         tol = -1.0
     with pytest.raises(_abi.RTMCError):
         nm.compress_table(x, y, tol, mk)
+
+
+def test_attach_rejects_materials_without_tables():
+    """rtmc_attach_lds_tables checks its materials before touching the device
+    (no GPU needed): a material without an inverse-CDF table is an argument
+    error, and nothing is attached."""
+    import ctypes as C
+    from radiativetransfermontecarlo_amd import _abi
+    lib = _abi.load()
+    mats = (_abi.Material * 2)()
+    mats[0].acoustics = 1
+    mats[1].acoustics = 0
+    h = C.c_void_p()
+    rc = lib.rtmc_attach_lds_tables(mats, 2, 1e-5, 0, C.byref(h), None)
+    assert rc == _abi.ERR_ARG and not h.value
+    assert "inverse-CDF table 0 is missing" in lib.rtmc_last_error().decode()
+    assert all(mats[i].lds[c].m == 0 for i in range(2) for c in range(4))
+    assert lib.rtmc_attach_lds_tables(mats, 0, 1e-5, 0, C.byref(h), None) == _abi.OK and not h.value
+    assert lib.rtmc_attach_lds_tables(mats, 1, -1.0, 0, C.byref(h), None) == _abi.ERR_ARG
+    assert lib.rtmc_attach_lds_tables(mats, 1, 1e-5, 3, C.byref(h), None) == _abi.ERR_ARG
