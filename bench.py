@@ -98,14 +98,33 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(config, precision):
-    """HBM bytes per launch of the tracing kernel from the committed counter summary, if any."""
+def traffic_from_profiles(config, precision, key="bytes_per_launch"):
+    """A figure of the tracing kernel's committed counter summary (HBM bytes per
+    launch by default), if any."""
     name = TRAFFIC.get((config, precision))
     p = os.path.join(ROOT, "profiles", name) if name else None
     if p and os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+            return json.load(f).get(key)
     return None
+
+
+def probe_atomics(E):
+    """The request rate of no-return global atomics (rtmc_probe_atomics) at random
+    words and random aligned word pairs over the histogram's own buffer (its
+    contents are overwritten): the ceiling of the time-major deposits."""
+    import ctypes as C
+    import torch
+    from radiativetransfermontecarlo_amd import _abi
+    lib = _abi.load()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {"bytes": int(E.numel()) * 4}
+    for kind, name in ((0, "u32_random_per_s"), (1, "u64_random_pairs_per_s")):
+        r = C.c_double()
+        _abi.check(lib.rtmc_probe_atomics(C.c_void_p(E.data_ptr()), int(E.numel()), kind, 2000, C.byref(r), stream),
+                   "rtmc_probe_atomics")
+        out[name] = r.value
+    return out
 
 
 WORKLOADS = {
@@ -193,10 +212,28 @@ def measure(config, precision, steps, warmup, particles=0, chunks=1, block=0, pp
            "dim": d, "acoustics": bool(material.acoustics), "frame": geometry.get('frame', 'spherical')}
     out["value"] = out["scatter_events"] / (out["ms_per_step"] * 1e-3)
     prob = (geometry, source, material, observation, O)
+    if config in (2, 3, 5):
+        out["probe"] = probe_atomics(E)
     del E
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     return out, prob
+
+
+def atomic_view(m, config, key):
+    """The histogram atomics' request rate against the probed ceiling (None
+    without a committed WRITE_SIZE summary or a probe)."""
+    wb = traffic_from_profiles(config, key, "write_bytes")
+    if not wb or not m.get("probe"):
+        return None
+    req = wb / 32.0 / (m["kernel_ms"] * 1e-3)
+    pr = m["probe"]
+    return {"requests_per_launch": wb / 32.0, "requests_per_s": req,
+            "probe_u32_random_per_s": pr["u32_random_per_s"],
+            "probe_u64_random_pairs_per_s": pr["u64_random_pairs_per_s"], "probe_bytes": pr["bytes"],
+            "frac_of_u32_probe": req / pr["u32_random_per_s"],
+            "definition": "32-byte write requests of the histogram atomics (committed WRITE_SIZE summary) over the "
+                          "kernel's time, against rtmc_probe_atomics over the histogram's buffer in the same run"}
 
 
 def roofline(m, config, precision, variant=""):
@@ -207,8 +244,10 @@ def roofline(m, config, precision, variant=""):
         k_gbs = alg / (m["kernel_ms"] * 1e-3) / 1e9
         s_bytes = alg + 2 * m["histogram_bytes"]
         s_gbs = s_bytes / (m["ms_per_step"] * 1e-3) / 1e9
+        atomic = atomic_view(m, config, precision + variant)
         return {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": k_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(config, precision + variant),
+                "atomic_view": atomic,
                 "kernel": "k_trace", "kernel_ms": m["kernel_ms"],
                 "algorithmic_unit": "kernel-only: 8 B per histogram deposit (4 B read + 4 B write of the counter) "
                                     "x deposits per launch, over the kernel's duration",
@@ -230,6 +269,7 @@ def roofline(m, config, precision, variant=""):
         alg = BYTES_PER_DEPOSIT * m["deposits"]
         out["hbm_view"] = {"achieved": alg / (m["kernel_ms"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": alg / (m["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out["atomic_view"] = atomic_view(m, config, precision + variant)
     return out
 
 
