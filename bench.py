@@ -412,7 +412,8 @@ def run_atomic_probe():
     buf = torch.zeros(words_max, dtype=torch.int32, device="cuda")
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     rows = []
-    for kind, name in enumerate(("u32 random", "u64 random pairs", "u32 runs of 2", "u32 runs of 8")):
+    for kind, name in enumerate(("u32 random", "u64 random pairs", "u32 runs of 2", "u32 runs of 8",
+                                 "u64 random pairs, returning")):
         for words in (1 << 20, 1 << 25, 1 << 28, words_max):
             r = C.c_double()
             _abi.check(lib.rtmc_probe_atomics(C.c_void_p(buf.data_ptr()), words, kind, 2000, C.byref(r), stream),

@@ -547,7 +547,7 @@ int rtmc_lc_derivatives(const rtmc_material_params* params, int32_t n, int64_t n
  * fused kernel's deposits when the histogram is in global memory.  buf:
  * device uint32[words] (contents are changed).  kind 0: uint32 at random
  * words; 1: uint64 at random aligned pairs; 2 / 3: runs of 2 / 8 consecutive
- * words from one lane.  Synchronises `stream`. */
+ * words from one lane; 4: as 1, returning the old values.  Synchronises `stream`. */
 int rtmc_probe_atomics(uint32_t* buf, int64_t words, int32_t kind, int64_t iters,
                        double* requests_per_s, void* stream);
 

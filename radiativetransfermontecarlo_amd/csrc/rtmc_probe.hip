@@ -14,13 +14,20 @@ __device__ __forceinline__ uint64_t xorshift(uint64_t x) {
 }
 
 // kind 0: uint32 at random words; 1: uint64 at random aligned pairs;
-// 2 / 3: a random word then the next 1 / 7 words from the same lane (runs of 2 / 8)
+// 2 / 3: a random word then the next 1 / 7 words from the same lane (runs of 2 / 8);
+// 4: uint64 at random aligned pairs returning the old value (the values are
+// summed and the sum decides a store at the end, so every return is waited for)
 __global__ __launch_bounds__(256) void k_probe(uint32_t* __restrict__ E, uint64_t words, int kind, int64_t iters) {
   uint64_t x = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 0x9E3779B97F4A7C15ull + 7u;
   const int run = kind == 2 ? 2 : 8;
   uint64_t base = 0;
+  unsigned long long acc = 0;
   for (int64_t i = 0; i < iters; ++i) {
-    if (kind == 0) {
+    if (kind == 4) {
+      x = xorshift(x);
+      acc += __hip_atomic_fetch_add((unsigned long long*)E + (x % (words / 2)), 0x100000001ull, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    } else if (kind == 0) {
       x = xorshift(x);
       __hip_atomic_fetch_add(E + (x % words), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (kind == 1) {
@@ -35,6 +42,7 @@ __global__ __launch_bounds__(256) void k_probe(uint32_t* __restrict__ E, uint64_
       __hip_atomic_fetch_add(E + base + (i % run), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+  if (kind == 4 && acc == 0x5A5A5A5A5A5A5A5Aull) E[0] = 1u;  // keeps the returned values live
 }
 
 }  // namespace
@@ -47,7 +55,7 @@ extern "C" int rtmc_probe_atomics(uint32_t* buf, int64_t words, int32_t kind, in
   clear_error();
   RTMC_REQUIRE(buf != nullptr && requests_per_s != nullptr, "buf and requests_per_s must be given");
   RTMC_REQUIRE(words >= 16, "words must be >= 16");
-  RTMC_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0..3");
+  RTMC_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0..4");
   RTMC_REQUIRE(iters >= 1 && iters <= 100000, "iters must be in [1, 100000]");
   hipStream_t st = (hipStream_t)stream;
   int dev = 0, ncu = 0;

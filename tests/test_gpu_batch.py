@@ -337,7 +337,7 @@ def test_trace_timing_and_atomic_probe(gpu):
     buf = torch.zeros(1 << 20, dtype=torch.int32, device='cuda')
     r = C.c_double()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    for kind in range(4):
+    for kind in range(5):
         _abi.check(lib.rtmc_probe_atomics(C.c_void_p(buf.data_ptr()), 1 << 20, kind, 10, C.byref(r), stream),
                    "probe")
         assert 1e8 < r.value < 1e13
