@@ -253,6 +253,35 @@ def test_histograms_do_not_depend_on_launch_configuration(gpu):
     assert torch.equal(B.forward()[0], E0)
 
 
+@pytest.mark.parametrize("case", ["acoustic3d", "acoustic3d_nearest", "elastic2d"])
+def test_global_and_lds_guides_give_identical_histograms(gpu, monkeypatch, case):
+    """The inverse-CDF lookup finds the same interval and applies the same formula
+    through the global guide (default) and through the LDS guide
+    (RTMC_GLOBAL_GUIDE=0): identical histograms and counters, with linear and
+    nearest extrapolation, one table (acoustic) and four (elastic)."""
+    if case.startswith("acoustic3d"):
+        geometry, source, base, observation = cases.config1()
+        observation = dict(observation, time=np.arange(0, 1.0 + 1e-9, 0.02))
+        kt = 0.0 if case.endswith("nearest") else 1e-8   # 0: nearest extrapolation
+        m = rb.prepare_material(base, 3, c=0.08, lc=0.2, Q=300.0, knot_threshold=kt)
+        O = initializeObservation(geometry, True, observation, 1)
+        n = 200_000
+    else:
+        geometry = {'dimension': 2, 'frame': 'cartesian'}
+        source = {'numberParticles': 1, 'lambda': 0.01, 'position': [0, 0]}
+        e = np.linspace(-3, 3, 25)
+        observation = {'x': e, 'y': e, 'time': np.arange(0, 0.8 + 1e-9, 0.05)}
+        m = cases.config2()[2]
+        m = MaterialClass.prepareSigma(m, 2)
+        O = initializeObservation(geometry, False, observation, 1)
+        n = 100_000
+    ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, seed=53, device='cuda')
+    assert int(ref.sum()) > 0
+    monkeypatch.setenv("RTMC_GLOBAL_GUIDE", "0")
+    E, c, _k = solver.forward_histogram(geometry, source, m, O, n, seed=53, device='cuda')
+    assert torch.equal(E, ref) and torch.equal(c, rc)
+
+
 @pytest.mark.parametrize("tmax", [0.8, 0.75])   # 17 and 16 observation times (odd / even)
 def test_time_major_scratch_equals_direct_atomics(gpu, tmax):
     """hist_mode 3 (time-major scratch, paired 64-bit atomics, transpose-add)
