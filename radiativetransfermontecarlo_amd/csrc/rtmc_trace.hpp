@@ -659,7 +659,6 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
   }
   const BinSetup& bs = s_bs;  // read from LDS where used (wave-uniform values, no registers held)
   const int lane = threadIdx.x & 63;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   while (true) {
     if (threadIdx.x == 0) {
@@ -896,7 +895,10 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             }
           }
           if (need) {  // nm != 0: this step births
-            const int r = __popcll(nm & lt_mask);
+            // rank among the birthing lanes: set bits of nm below this lane (mbcnt
+            // counts them without a per-lane mask held in two registers)
+            const int r = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(nm >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
             int64_t id = -1;
             if (r < L) id = pool_next + r;
             else if (r - L < nb1 - nb0) id = nb0 + (r - L);
