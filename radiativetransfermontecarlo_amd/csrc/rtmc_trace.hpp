@@ -840,6 +840,24 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
     int dbin = 0;                   // bin of the direction variable for the current direction
     int nsc = 0;                    // adjoint: scatterings of the particle so far
     double gsum = 0.0;              // adjoint: sum of dlogsigma_dlc(theta) - dlogSigma_dlc over them
+    // adjoint: sum of w and of w * it over the deposits since the particle's last
+    // scattering, whose scores share nsc and gsum and are linear in it
+    double aw = 0.0, awi = 0.0;
+    // adds those deposits' w A, w gsum and w k to the lane's sums (before nsc or
+    // gsum change, and when the lane is done): sub-stepped A = nsc (lam/expm1(lam)
+    // + lam) - it nsub lam and k = it nsub, event-driven A = nsc - it dt / mft
+    auto fold = [&]() {
+      if (!ADJ) return;
+      if (QUANT) {
+        gaA += (double)nsc * (P.lam_over_expm1 + P.lamS[0]) * aw - (double)P.nsub * P.lamS[0] * awi;
+        gaK += (double)P.nsub * awi;
+      } else {
+        gaA += (double)nsc * aw - (p.dt_obs / M.mftP) * awi;
+      }
+      gaG += gsum * aw;
+      aw = 0.0;
+      awi = 0.0;
+    };
     // wave-uniform pool of particle indices fetched from the problem's counter
     int64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
@@ -913,6 +931,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               it = 1;
               has = true;
               need = false;
+              fold();  // the lane's previous particle
               nsc = 0;
               gsum = 0.0;
               if (QUANT) {
@@ -987,6 +1006,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
                   rng.scatter4<R, AC>(uphi, uth, uconv, ugs);
                   double th = scatter_angles_t<DIM, R, decltype(lookup), true>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
                   if (ADJ) {
+                    fold();
                     nsc += 1;
                     gsum += glc_eval(P, th) - P.dlogS;
                   }
@@ -1028,6 +1048,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
             rng.scatter4<R, AC>(uphi, uth, uconv, uexp);
             double th = scatter_angles_t<DIM, R, decltype(lookup), true>(M, AC, d, q, isP, uphi < 0.5, uphi, uth, uconv, lookup);
             if (ADJ) {
+              fold();
               nsc += 1;
               gsum += glc_eval(P, th) - P.dlogS;
             }
@@ -1139,20 +1160,20 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
               else __hip_atomic_fetch_add(PE + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
               // score of the trajectory up to this observation: A = d log L / d log lam
-              // (scattering tests), k = removal tests survived
-              double A, k;
-              if (QUANT) {
-                k = (double)it * (double)P.nsub;
-                A = (double)nsc * P.lam_over_expm1 - (k - (double)nsc) * P.lamS[0];
-              } else {
-                k = 0.0;
-                A = (double)nsc - (double)it * p.dt_obs / M.mftP;
-              }
+              // (scattering tests), k = removal tests survived; summed through aw and
+              // awi (fold)
               const double w = LDSH ? (double)sw[idx] : (double)P.w[idx];
-              gaA += w * A;
-              gaG += w * gsum;
-              gaK += w * k;
+              aw += w;
+              awi += w * (double)it;
               if (P.dhist) {
+                double A, k;
+                if (QUANT) {
+                  k = (double)it * (double)P.nsub;
+                  A = (double)nsc * P.lam_over_expm1 - (k - (double)nsc) * P.lamS[0];
+                } else {
+                  k = 0.0;
+                  A = (double)nsc - (double)it * p.dt_obs / M.mftP;
+                }
                 unsafeAtomicAdd(P.dhist + idx, A * P.two_over_c);
                 unsafeAtomicAdd(P.dhist + P.dhist_stride + idx, A * P.dlogS + gsum);
                 if (P.kq != 0.0) unsafeAtomicAdd(P.dhist + 2 * P.dhist_stride + idx, k * P.kq);
@@ -1171,6 +1192,7 @@ __global__ __launch_bounds__(HM == HM_LDS ? 1024 : 256) void k_trace(const Launc
 
     if (TMAJ && tpend >= 0) __hip_atomic_fetch_add(PT + tpend, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (RUNS3 && jrow >= 0) flush_run();
+    fold();
     __syncthreads();
     if (TMAJ && !ADJ) {
       for (int k = threadIdx.x; k < p.hot_slots; k += blockDim.x) {
