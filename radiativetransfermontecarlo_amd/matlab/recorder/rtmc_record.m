@@ -15,10 +15,11 @@ function rec = rtmc_record(refdir, name, N, outfile)
 %   outfile (MAT v7).  tests/test_matlab_replay.py replays such a file
 %   through librtmc: bins and counts exactly, particle states to 1e-12.
 %
-%   The draws are caught by shadows of rand, randn, randi and poissrnd that
-%   this folder holds and that are on the path only while the run lasts;
+%   The draws are caught by shadows of rand, randn, randi and poissrnd (the
+%   shadows subfolder), which this function puts on the path for the run only;
 %   each site is found by its source text, so a changed reference line stops
-%   the recorder with an error instead of recording the wrong site.
+%   the recorder with an error instead of recording the wrong site.  Put this
+%   folder (not shadows) on the path, or run from it.
 %
 %   Cases: 'readme3d' (3-D acoustic, spherical frame, with Q), 'elastic2d'
 %   (2-D elastic, Q), 'halfspace3d' (3-D elastic under a free surface z = 0:
@@ -32,11 +33,11 @@ function rec = rtmc_record(refdir, name, N, outfile)
 %                   fullfile(repo, 'tests', 'golden', 'matlab', 'readme3d.mat'));
 
 global RTMC_REC
-here = fileparts(mfilename('fullpath'));
+shadows = fullfile(fileparts(mfilename('fullpath')), 'shadows');
 addpath(refdir);
-% the toolbox poissrnd, taken before this folder's shadow goes on the path
+% the toolbox poissrnd, taken before the shadow goes on the path
 p = which('poissrnd', '-all');
-p = p(~startsWith(p, here));
+p = p(~startsWith(p, shadows));
 if isempty(p)
     error('rtmc_record:toolbox', 'poissrnd (Statistics and Machine Learning Toolbox) is required');
 end
@@ -56,8 +57,8 @@ end
 dt = mean(diff(t));
 
 RTMC_REC = struct('on', false, 'poissrnd', hpois, 'sites', rtmc_sites(refdir), 'call', []);
-addpath(here, '-begin');
-cleanup = onCleanup(@() rtmc_stop(here));
+addpath(shadows, '-begin');
+cleanup = onCleanup(@() rtmc_stop(shadows));
 RTMC_REC.on = true;
 
 rec_begin('init', 8, N);
@@ -150,10 +151,10 @@ if ~material.acoustics && isfield(geometry, 'bnd')
 end
 end
 
-function rtmc_stop(here)
+function rtmc_stop(shadows)
 global RTMC_REC
 RTMC_REC = [];
-rmpath(here);
+rmpath(shadows);
 end
 
 function [geometry, source, material, observation] = rtmc_case(name, N)

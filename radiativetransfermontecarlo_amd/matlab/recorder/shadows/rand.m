@@ -1,6 +1,6 @@
 function r = rand(varargin)
-%RAND  Recording shadow of the builtin rand, active only while rtmc_record
-%runs (this folder is on the path then): returns the builtin's values and,
+%RAND  Recording shadow of the builtin rand, on the path only while
+%rtmc_record runs: returns the builtin's values and,
 %when the caller is one of the reference's draw sites, stores them in
 %librtmc's replay layout.  Calls with a RandStream dispatch to its method and
 %are not seen here.

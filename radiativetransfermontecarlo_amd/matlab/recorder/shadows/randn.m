@@ -1,10 +1,10 @@
-function r = randi(varargin)
-%RANDI  Recording shadow of the builtin randi, active only while rtmc_record
-%runs (this folder is on the path then): returns the builtin's values and,
+function r = randn(varargin)
+%RANDN  Recording shadow of the builtin randn, on the path only while
+%rtmc_record runs: returns the builtin's values and,
 %when the caller is one of the reference's draw sites, stores them in
 %librtmc's replay layout.  Calls with a RandStream dispatch to its method and
 %are not seen here.
-r = builtin('randi', varargin{:});
+r = builtin('randn', varargin{:});
 s = rtmc_rec_site();
 if ~isempty(s)
     idx = evalin('caller', s.idx);
