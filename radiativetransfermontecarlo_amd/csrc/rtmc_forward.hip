@@ -181,6 +181,7 @@ struct Plan {
   int64_t tsize;  // HM_TMAJOR: words of one problem's time-major scratch
   bool tmaj_auto; // HM_TMAJOR chosen by default (falls back to HM_GLOBAL without memory)
   bool global_guide;  // build the global guide tables (else the LDS guide, if any)
+  int guide_finer;    // cell doublings of the global guide beyond one cell per knot (-1: automatic)
   int64_t run_particles;  // runs of three: particles per scratch copy (<= kRunMax)
   int64_t sub_words;      // runs of three: 64-bit words of one scratch copy
 };
@@ -392,6 +393,8 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // RTMC_GLOBAL_GUIDE=0 keeps the LDS guide, which takes what the histogram leaves.
   // Compressed tables in LDS need neither.
   pl.global_guide = getenv_int("RTMC_GLOBAL_GUIDE", 1) != 0 && lt_cap == 0;
+  pl.guide_finer = getenv_int("RTMC_GUIDE_FINER", -1);
+  RTMC_REQUIRE(pl.guide_finer >= -1 && pl.guide_finer <= 6, "RTMC_GUIDE_FINER must be -1 (automatic) or in [0, 6]");
   if (pl.global_guide || lt_cap > 0) guide = 0;
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
@@ -608,6 +611,24 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Ar
     for (int t = 0; t < pl.p.ntab; ++t) nmax = std::max<int64_t>(nmax, probs[i].M.inv[t].n);
   int lg = 10;
   while (lg < 22 && ((int64_t)1 << lg) < nmax) ++lg;
+  {
+    // More cells per knot when many particles share the tables: a lookup then
+    // brackets fewer knots (each search step is a dependent global read), for the
+    // price of building and caching a larger guide.  Automatic (-1): one doubling
+    // per factor two of particles beyond 64 per knot, at most 16 cells per knot;
+    // RTMC_GUIDE_FINER = k doubles k times.  The guides stay within 1 GiB (half the
+    // free no-alloc scratch).
+    int64_t parts = 0;
+    for (int i = 0; i < np; ++i) parts += probs[i].n;
+    int extra = pl.guide_finer;
+    if (extra < 0) {
+      extra = 0;
+      while (extra < 4 && parts >= ((int64_t)64 << (extra + 1)) * nmax) ++extra;
+    }
+    const size_t cap = ar.base ? ar.avail() / 2 : ((size_t)1 << 30);
+    while (extra > 0 && sizeof(int) * (((size_t)1 << (lg + extra)) + 1) * (size_t)pl.p.ntab * (size_t)np > cap) --extra;
+    lg += extra;
+  }
   const bool want_gg = pl.global_guide && pl.p.lt_cap == 0 && nmax <= ((int64_t)1 << 31) - 2;
   const size_t gcells = ((size_t)1 << lg) + 1;
   const size_t gbytes = sizeof(int) * gcells * (size_t)pl.p.ntab * (size_t)np;

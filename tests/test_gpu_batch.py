@@ -256,8 +256,8 @@ def test_histograms_do_not_depend_on_launch_configuration(gpu):
 @pytest.mark.parametrize("case", ["acoustic3d", "acoustic3d_nearest", "elastic2d"])
 def test_global_and_lds_guides_give_identical_histograms(gpu, monkeypatch, case):
     """The inverse-CDF lookup finds the same interval and applies the same formula
-    through the global guide (default) and through the LDS guide
-    (RTMC_GLOBAL_GUIDE=0): identical histograms and counters, with linear and
+    through the global guide (default), a finer global guide (RTMC_GUIDE_FINER)
+    and the LDS guide (RTMC_GLOBAL_GUIDE=0): identical histograms and counters, with linear and
     nearest extrapolation, one table (acoustic) and four (elastic)."""
     if case.startswith("acoustic3d"):
         geometry, source, base, observation = cases.config1()
@@ -277,6 +277,11 @@ def test_global_and_lds_guides_give_identical_histograms(gpu, monkeypatch, case)
         n = 100_000
     ref, rc, _k = solver.forward_histogram(geometry, source, m, O, n, seed=53, device='cuda')
     assert int(ref.sum()) > 0
+    # a finer global guide (8 times the cells)
+    monkeypatch.setenv("RTMC_GUIDE_FINER", "3")
+    E, c, _k = solver.forward_histogram(geometry, source, m, O, n, seed=53, device='cuda')
+    assert torch.equal(E, ref) and torch.equal(c, rc)
+    monkeypatch.delenv("RTMC_GUIDE_FINER")
     monkeypatch.setenv("RTMC_GLOBAL_GUIDE", "0")
     E, c, _k = solver.forward_histogram(geometry, source, m, O, n, seed=53, device='cuda')
     assert torch.equal(E, ref) and torch.equal(c, rc)

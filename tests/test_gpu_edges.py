@@ -4,6 +4,7 @@ and the host-side argument checks (they return an error status and message
 before anything is launched)."""
 import ctypes as C
 import math
+import re
 
 import numpy as np
 import pytest
@@ -146,6 +147,7 @@ def test_null_pointers_are_rejected(gpu):
     ("RTMC_HOT_SLOTS", "3000", "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]"),
     ("RTMC_RUN_PARTICLES", "0", "RTMC_RUN_PARTICLES must be in [1, 2097151]"),
     ("RTMC_RUN_PARTICLES", "2097152", "RTMC_RUN_PARTICLES must be in [1, 2097151]"),
+    ("RTMC_GUIDE_FINER", "7", "RTMC_GUIDE_FINER must be -1 (automatic) or in [0, 6]"),
 ])
 def test_invalid_tuning_variables_are_rejected_before_launch(gpu, monkeypatch, var, value, msg):
     """The time-major tuning variables are checked when the launch is planned:
@@ -158,7 +160,7 @@ def test_invalid_tuning_variables_are_rejected_before_launch(gpu, monkeypatch, v
     O = initializeObservation(geometry, False, observation, 1)
     monkeypatch.setenv(var, value)
     E = torch.zeros(O.size, dtype=torch.int32, device='cuda')
-    with pytest.raises(RuntimeError, match=msg.replace("[", r"\[").replace("]", r"\]")):
+    with pytest.raises(RuntimeError, match=re.escape(msg)):
         solver.forward_histogram(geometry, source, m, O, 1000, seed=1, device='cuda', hist_mode=3, E=E)
     torch.cuda.synchronize()
     assert int(E.abs().sum()) == 0
