@@ -612,14 +612,15 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Ar
   int lg = 10;
   while (lg < 22 && ((int64_t)1 << lg) < nmax) ++lg;
   {
-    // More cells per knot when many particles share the tables: a lookup then
+    // More cells per knot when many particles share a table: a lookup then
     // brackets fewer knots (each search step is a dependent global read), for the
     // price of building and caching a larger guide.  Automatic (-1): one doubling
-    // per factor two of particles beyond 64 per knot, at most 16 cells per knot;
-    // RTMC_GUIDE_FINER = k doubles k times.  The guides stay within 1 GiB (half the
-    // free no-alloc scratch).
+    // per factor two of particles per problem beyond 64 per knot, at most 16 cells
+    // per knot; RTMC_GUIDE_FINER = k doubles k times.  The guides stay within 1 GiB
+    // (half the free no-alloc scratch).
     int64_t parts = 0;
     for (int i = 0; i < np; ++i) parts += probs[i].n;
+    parts /= std::max(1, np);
     int extra = pl.guide_finer;
     if (extra < 0) {
       extra = 0;
