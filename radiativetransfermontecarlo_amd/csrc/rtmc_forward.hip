@@ -463,10 +463,14 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // elastic and half-space cases), 40 at 0.35 (lc = 0.05, f = 40, Q = 300:
   // 193.7 ms per 1e8 particles at 48, 190.3 at 40, 207.7 at 56).
   // FP32 flight runs twice the wavefronts per SIMD and prefers 48 there (1e9-particle
-  // case per 1e8 particles: 185 ms at 48 against 196 at 40).
+  // case per 1e8 particles: 185 ms at 48 against 196 at 40).  With the finer global
+  // guide the FP64 full-table lookups of the time-major kernel prefer larger event
+  // steps, whose many lanes overlap their dependent reads: 1e9-particle kernel
+  // 1816 ms at 44, 1789 at 48, 1764 at 50, 1648 at 52, 1686 at 54, 1914 at 60; with
+  // the tables in LDS 40 stays best (1536 ms; 1597 at 48).
   const bool f32 = precision == RTMC_PREC_FP32_FLIGHT;
-  p.batch = event_batch > 0 ? event_batch
-                            : (ev_per_obs >= 0.15 ? (f32 ? 48 : 40) : (ev_per_obs >= 0.02 ? 32 : 16));
+  const int heavy = f32 ? 48 : ((lt_cap == 0 && pl.hm == HM_TMAJOR) ? 52 : 40);
+  p.batch = event_batch > 0 ? event_batch : (ev_per_obs >= 0.15 ? heavy : (ev_per_obs >= 0.02 ? 32 : 16));
   if (event_batch == 0) {
     if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
   }
