@@ -52,20 +52,24 @@ __global__ __launch_bounds__(256) void k_tmajor_add(const uint32_t* __restrict__
     if (k < 0) return;  // the tracing kernel deposited nothing in the scratch
     base = k * dir_stride;
   }
+  // the tile covers scratch columns [c0, c0 + 64) of 64 bins, i.e. times c0 + shift + c;
+  // rows start at even words (nt_pad is even), so columns are read in aligned pairs
   const int64_t sp0 = (int64_t)blockIdx.x * 64;
-  const int it0 = blockIdx.y * 64;
+  const int c0 = blockIdx.y * 64;
   const int64_t m = blockIdx.z;
-  for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
-    const int r = k >> 6, c = k & 63;
+  for (int k = threadIdx.x; k < 64 * 32; k += blockDim.x) {
+    const int r = k >> 5, c = (k & 31) * 2;
     const int64_t sp = sp0 + r;
-    const int it = it0 + c;
-    tile[r][c] = (sp < tsp && it < nt && it >= shift) ? T[(sp + m * tsp) * nt_pad + it - shift] : 0u;
+    uint2 v = make_uint2(0u, 0u);
+    if (sp < tsp && c0 + c < nt_pad) v = *(const uint2*)(T + (sp + m * tsp) * nt_pad + c0 + c);
+    tile[r][c] = v.x;
+    tile[r][c + 1] = v.y;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
     const int c = k >> 6, r = k & 63;
     const int64_t sp = sp0 + r;
-    const int it = it0 + c;
+    const int it = c0 + c + shift;
     const uint32_t v = tile[r][c];
     if (v && sp < tsp && it < nt) E[base + sp + (int64_t)it * ts + m * ms] += v;
   }
@@ -756,7 +760,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Ar
     }
   } else if (rc == RTMC_OK && tbuf) {
     const int64_t ts = pl.p.time_stride;
-    dim3 grid((unsigned)(((int64_t)pl.p.tsp + 63) / 64), (unsigned)((pl.p.n_times + 63) / 64), (unsigned)pl.n_modes);
+    dim3 grid((unsigned)(((int64_t)pl.p.tsp + 63) / 64), (unsigned)((pl.p.nt_pad + 63) / 64), (unsigned)pl.n_modes);
     for (int i = 0; i < np && rc == RTMC_OK; ++i) {
       for (int sh = 0; sh <= (pl.p.odd_pairs ? 1 : 0); ++sh) {
         hipLaunchKernelGGL(k_tmajor_add, grid, dim3(256), 0, st, (const uint32_t*)(sh ? probs[i].T2 : probs[i].T),
