@@ -536,15 +536,35 @@ __device__ __forceinline__ void rotate_renorm(T d[3], T q[3], T th) {
   }
 }
 
-// rotate_renorm with one division per vector (the fused kernel)
+// 1/sqrt(s) for a normal s > 0: the hardware estimate (v_rsq_f64, good to
+// about 2^-23 relative) refined by two Newton steps y <- y (3 - s y^2) / 2,
+// each of which squares the relative error, so the result is within a few
+// ulp of the rounded 1/sqrt(s), without the division and the correctly
+// rounded square root.
+__device__ __forceinline__ double rsqrt_nr(double s) {
+  double y = __builtin_amdgcn_rsq(s);
+  const double hs = 0.5 * s;
+  y = y * fma(-hs, y * y, 1.5);
+  y = y * fma(-hs, y * y, 1.5);
+  return y;
+}
+// FP32: v_rsq_f32 (1 ulp) refined once
+__device__ __forceinline__ float rsqrt_nr(float s) {
+  float y = __builtin_amdgcn_rsqf(s);
+  return y * fmaf(-0.5f * s, y * y, 1.5f);
+}
+
+// rotate_renorm with the norms from rsqrt_nr (the fused kernel: its vectors
+// are unit to a few ulp, so this differs from the division by the correctly
+// rounded norm by a few ulp)
 template <typename T>
 __device__ __forceinline__ void rotate_renorm_fast(T d[3], T q[3], T th) {
   T st, ct;
-  sc(th, &st, &ct);
+  sc2pi(th * (T)(0.5 / kPi), &st, &ct);
   T nd[3], np[3];
   for (int k = 0; k < 3; ++k) nd[k] = ct * d[k] + st * q[k];
   for (int k = 0; k < 3; ++k) np[k] = -st * d[k] + ct * q[k];
-  const T id = T(1) / sqrt(sumsq3(nd)), ip = T(1) / sqrt(sumsq3(np));
+  const T id = rsqrt_nr(sumsq3(nd)), ip = rsqrt_nr(sumsq3(np));
   for (int k = 0; k < 3; ++k) {
     d[k] = nd[k] * id;
     q[k] = np[k] * ip;
