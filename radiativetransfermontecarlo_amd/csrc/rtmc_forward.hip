@@ -471,9 +471,12 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // guide the FP64 full-table lookups of the time-major kernel prefer larger event
   // steps, whose many lanes overlap their dependent reads: 1e9-particle kernel
   // 1816 ms at 44, 1789 at 48, 1764 at 50, 1648 at 52, 1686 at 54, 1914 at 60; with
-  // the tables in LDS 40 stays best (1536 ms; 1597 at 48).
+  // the tables in LDS 40 stays best (1536 ms; 1597 at 48).  After the polar angle's
+  // sin/cos moved to sincospi, one box, two runs each: 1731 at 50, 1728 at 51, 1650
+  // at 52, 1638 at 53, 1661 at 54 (and 1454 at 40, 1449 at 36, 1459 at 32 with the
+  // LDS tables).
   const bool f32 = precision == RTMC_PREC_FP32_FLIGHT;
-  const int heavy = f32 ? 48 : ((lt_cap == 0 && pl.hm == HM_TMAJOR) ? 52 : 40);
+  const int heavy = f32 ? 48 : ((lt_cap == 0 && pl.hm == HM_TMAJOR) ? 53 : 40);
   p.batch = event_batch > 0 ? event_batch : (ev_per_obs >= 0.15 ? heavy : (ev_per_obs >= 0.02 ? 32 : 16));
   if (event_batch == 0) {
     if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
