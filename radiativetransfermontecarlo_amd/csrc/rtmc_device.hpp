@@ -554,9 +554,11 @@ __device__ __forceinline__ float rsqrt_nr(float s) {
   return y * fmaf(-0.5f * s, y * y, 1.5f);
 }
 
-// rotate_renorm with the norms from rsqrt_nr (the fused kernel: its vectors
-// are unit to a few ulp, so this differs from the division by the correctly
-// rounded norm by a few ulp)
+// rotate_renorm for the fused kernel: sin and cos of theta through sincospi
+// of theta/pi (the reduction by whole turns is exact; theta times the rounded
+// 1/(2 pi) moves the angle by about 1e-15 rad), the norms from rsqrt_nr (the
+// vectors are unit to a few ulp, so the result differs from the division by
+// the correctly rounded norm by a few ulp)
 template <typename T>
 __device__ __forceinline__ void rotate_renorm_fast(T d[3], T q[3], T th) {
   T st, ct;
