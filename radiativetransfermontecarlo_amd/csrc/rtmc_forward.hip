@@ -402,15 +402,18 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   if (pl.global_guide || lt_cap > 0) guide = 0;
   while (guide > 0 && edge_bytes + guide_bytes(guide) + (ldsh ? hist_bytes : 0) > (size_t)max_smem) guide >>= 1;
   if (guide < 64) guide = 0;
-  // time-major mode: a cache of hot counters, 8 B a slot: 4096 slots (32 KB), or
-  // 2048 with FP32 flight, whose instances use few registers and would lose half
-  // their work-groups per CU to a larger cache (1e9-particle case: 1.96 s with
-  // 2048, 2.19 s with 4096; FP64 3-D half-space 381.6 -> 375.8 ms with 4096, its
-  // occupancy being set by registers).  The LDS guide, if any, shrinks to keep the
-  // work-group within 40 KB.
+  // time-major mode: a cache of hot counters, 8 B a slot: 4096 slots (32 KB).  FP32
+  // flight had used 2048 while its kernel was bound by its own work, as its
+  // instances use few registers and lose half their work-groups per CU to the
+  // larger cache (1e9-particle case: 1.96 s with 2048, 2.19 s with 4096); since
+  // the polar angle's sin/cos went through sincospi that kernel waits on the
+  // atomics instead, and the requests the larger cache saves pay (1637 -> 1626 ms,
+  // means of four runs each in ABBA order).  FP64 3-D half-space 381.6 -> 375.8 ms with
+  // 4096, its occupancy being set by registers.  The LDS guide, if any, shrinks to
+  // keep the work-group within 40 KB.
   int hot = 0;
   if (tmaj) {
-    hot = (dim == 2 || precision != RTMC_PREC_FP32_FLIGHT) ? 4096 : 2048;
+    hot = 4096;
     if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
     RTMC_REQUIRE(hot == 0 || (hot >= 2 && hot <= (1 << 13) && (hot & (hot - 1)) == 0),
                  "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]");
@@ -474,7 +477,9 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // the tables in LDS 40 stays best (1536 ms; 1597 at 48).  After the polar angle's
   // sin/cos moved to sincospi, one box, two runs each: 1731 at 50, 1728 at 51, 1650
   // at 52, 1638 at 53, 1661 at 54 (and 1454 at 40, 1449 at 36, 1459 at 32 with the
-  // LDS tables).
+  // LDS tables); 52 against 53 in ABBA order, four runs each: 1644 and 1636 ms, 53
+  // losing less where the scratch's placement takes fewer atomics per second (1640
+  // against 1671 ms).
   const bool f32 = precision == RTMC_PREC_FP32_FLIGHT;
   const int heavy = f32 ? 48 : ((lt_cap == 0 && pl.hm == HM_TMAJOR) ? 53 : 40);
   p.batch = event_batch > 0 ? event_batch : (ev_per_obs >= 0.15 ? heavy : (ev_per_obs >= 0.02 ? 32 : 16));
