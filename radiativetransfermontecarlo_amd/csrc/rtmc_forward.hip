@@ -479,9 +479,10 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // at 52, 1638 at 53, 1661 at 54 (and 1454 at 40, 1449 at 36, 1459 at 32 with the
   // LDS tables); 52 against 53 in ABBA order, four runs each: 1644 and 1636 ms, 53
   // losing less where the scratch's placement takes fewer atomics per second (1640
-  // against 1671 ms).
+  // against 1671 ms).  The compressed-table instances take 36 (1449 against 1455 ms
+  // at 40, ABBA order, four runs each).
   const bool f32 = precision == RTMC_PREC_FP32_FLIGHT;
-  const int heavy = f32 ? 48 : ((lt_cap == 0 && pl.hm == HM_TMAJOR) ? 53 : 40);
+  const int heavy = f32 ? 48 : (lt_cap > 0 ? 36 : (pl.hm == HM_TMAJOR ? 53 : 40));
   p.batch = event_batch > 0 ? event_batch : (ev_per_obs >= 0.15 ? heavy : (ev_per_obs >= 0.02 ? 32 : 16));
   if (event_batch == 0) {
     if (const char* b = getenv("RTMC_EVENT_BATCH")) p.batch = atoi(b);
