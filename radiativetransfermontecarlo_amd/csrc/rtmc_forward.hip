@@ -411,9 +411,16 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // means of four runs each in ABBA order).  FP64 3-D half-space 381.6 -> 375.8 ms with
   // 4096, its occupancy being set by registers.  The LDS guide, if any, shrinks to
   // keep the work-group within 40 KB.
+  // 3-D pairs with FP64 flight and the full tables (`wide`): work-groups of 1024
+  // threads, one per CU (registers allow 16 wavefronts), sharing one cache of 8192
+  // slots instead of four work-groups with 4096 each; the 1e9-particle kernel
+  // 1636 -> 1618 ms (1621 with 512 threads, 1631 with 512 threads and 4096
+  // slots), ABBA order.
+  const bool wide = tmaj && dim == 3 && lt_cap == 0 && !p.runs3 && precision != RTMC_PREC_FP32_FLIGHT &&
+                    (block_size == 0 || block_size >= 512);
   int hot = 0;
   if (tmaj) {
-    hot = 4096;
+    hot = wide ? 8192 : 4096;
     if (const char* h = getenv("RTMC_HOT_SLOTS")) hot = atoi(h);
     RTMC_REQUIRE(hot == 0 || (hot >= 2 && hot <= (1 << 13) && (hot & (hot - 1)) == 0),
                  "RTMC_HOT_SLOTS must be 0 or a power of two in [2, 8192]");
@@ -446,9 +453,10 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   RTMC_REQUIRE(pl.smem <= (size_t)max_smem, "%zu bytes of LDS per work-group exceed the %d available beside the "
                "kernel's own", pl.smem, max_smem);
 
-  unsigned block = block_size > 0 ? (unsigned)block_size : (ldsh ? 1024u : 256u);
-  RTMC_REQUIRE(block % 64 == 0 && block <= (ldsh ? 1024u : 256u),
-               "block_size must be a multiple of 64 and <= 256 (<= 1024 with the LDS histogram)");
+  unsigned block = block_size > 0 ? (unsigned)block_size : ((ldsh || wide) ? 1024u : 256u);
+  RTMC_REQUIRE(block % 64 == 0 && block <= ((ldsh || (tmaj && dim == 3 && lt_cap == 0 && !p.runs3)) ? 1024u : 256u),
+               "block_size must be a multiple of 64 and <= 256 (<= 1024 with the LDS histogram, or with 3-D "
+               "time-major pairs and the full angle tables)");
   // resident work-groups per CU as LDS allows (the whole work-group's LDS, static part included)
   const int64_t lds_cu = (int64_t)max_smem + kTraceStaticLds, lds_wg = (int64_t)pl.smem + kTraceStaticLds;
   int64_t per_cu;
@@ -594,6 +602,12 @@ size_t tmajor_words(const Plan& pl, const std::vector<ProbDev>& probs, std::vect
 
 // the time-major placement is dropped: direct global atomics need no scratch
 void drop_tmajor(Plan& pl) {
+  // the global-atomics instances take at most 256 threads a work-group: the same
+  // threads in more, smaller work-groups
+  if (pl.block > 256) {
+    pl.grid *= pl.block / 256;
+    pl.block = 256;
+  }
   pl.hm = HM_GLOBAL;
   pl.p.hot_slots = 0;
   pl.p.tcomp = 0;
