@@ -411,13 +411,12 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // means of four runs each in ABBA order).  FP64 3-D half-space 381.6 -> 375.8 ms with
   // 4096, its occupancy being set by registers.  The LDS guide, if any, shrinks to
   // keep the work-group within 40 KB.
-  // 3-D pairs with FP64 flight and the full tables (`wide`): work-groups of 1024
-  // threads, one per CU (registers allow 16 wavefronts), sharing one cache of 8192
-  // slots instead of four work-groups with 4096 each; the 1e9-particle kernel
-  // 1636 -> 1618 ms (1621 with 512 threads, 1631 with 512 threads and 4096
-  // slots), ABBA order.
-  const bool wide = tmaj && dim == 3 && lt_cap == 0 && !p.runs3 && precision != RTMC_PREC_FP32_FLIGHT &&
-                    (block_size == 0 || block_size >= 512);
+  // 3-D pairs with the full tables (`wide`): work-groups of 1024 threads, one per
+  // CU (16 wavefronts, what registers or LDS allowed before too), sharing one
+  // cache of 8192 slots instead of four work-groups with 4096 each; the
+  // 1e9-particle kernel 1636 -> 1618 ms (1621 with 512 threads, 1631 with 512
+  // threads and 4096 slots), with FP32 flight 1628 -> 1609 ms, ABBA order.
+  const bool wide = tmaj && dim == 3 && lt_cap == 0 && !p.runs3 && (block_size == 0 || block_size >= 512);
   int hot = 0;
   if (tmaj) {
     hot = wide ? 8192 : 4096;
