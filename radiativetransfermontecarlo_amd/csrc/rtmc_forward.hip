@@ -416,7 +416,9 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
   // cache of 8192 slots instead of four work-groups with 4096 each; the
   // 1e9-particle kernel 1636 -> 1618 ms (1621 with 512 threads, 1631 with 512
   // threads and 4096 slots), with FP32 flight 1628 -> 1609 ms, ABBA order.
-  const bool wide = tmaj && dim == 3 && lt_cap == 0 && !p.runs3 && (block_size == 0 || block_size >= 512);
+  // 2-D runs of three too: their particles share a plane, so more of their
+  // counters are hot (2-D elastic kernel 40.96 -> 37.58 ms, ABBA order).
+  const bool wide = tmaj && lt_cap == 0 && (dim == 3) != (p.runs3 != 0) && (block_size == 0 || block_size >= 512);
   int hot = 0;
   if (tmaj) {
     hot = wide ? 8192 : 4096;
@@ -453,7 +455,7 @@ int make_plan(const rtmc_geometry& g, const rtmc_bins& B, int n_times, double dt
                "kernel's own", pl.smem, max_smem);
 
   unsigned block = block_size > 0 ? (unsigned)block_size : ((ldsh || wide) ? 1024u : 256u);
-  RTMC_REQUIRE(block % 64 == 0 && block <= ((ldsh || (tmaj && dim == 3 && lt_cap == 0 && !p.runs3)) ? 1024u : 256u),
+  RTMC_REQUIRE(block % 64 == 0 && block <= ((ldsh || (tmaj && lt_cap == 0 && (dim == 3) != (p.runs3 != 0))) ? 1024u : 256u),
                "block_size must be a multiple of 64 and <= 256 (<= 1024 with the LDS histogram, or with 3-D "
                "time-major pairs and the full angle tables)");
   // resident work-groups per CU as LDS allows (the whole work-group's LDS, static part included)
@@ -600,13 +602,17 @@ size_t tmajor_words(const Plan& pl, const std::vector<ProbDev>& probs, std::vect
 }
 
 // the time-major placement is dropped: direct global atomics need no scratch
-void drop_tmajor(Plan& pl) {
-  // the global-atomics instances take at most 256 threads a work-group: the same
-  // threads in more, smaller work-groups
+// The same threads in work-groups of 256, for instances that take no more (the
+// global-atomics ones, and the 2-D pairs that replace runs of three).
+void narrow_block(Plan& pl) {
   if (pl.block > 256) {
     pl.grid *= pl.block / 256;
     pl.block = 256;
   }
+}
+
+void drop_tmajor(Plan& pl) {
+  narrow_block(pl);
   pl.hm = HM_GLOBAL;
   pl.p.hot_slots = 0;
   pl.p.tcomp = 0;
@@ -684,6 +690,7 @@ int run_plan(Plan& pl, std::vector<ProbDev>& probs, int sink, hipStream_t st, Ar
       // the scratch array is too small for runs of three: pairs, then pairs from even times only
       if (pl.p.runs3) {
         pl.p.runs3 = 0;
+        if (pl.dim == 2) narrow_block(pl);
         pl.p.odd_pairs = 1;
         tbytes = sizeof(uint32_t) * tmajor_words(pl, probs, &nsub, &toff);
       }

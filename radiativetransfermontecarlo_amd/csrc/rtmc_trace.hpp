@@ -616,7 +616,7 @@ static_assert(sizeof(BinSetup) + 8 * sizeof(double) + 2 * sizeof(int) + 32 <= (s
 // LT: scattering angles from the compressed tables held in LDS (LaunchParams::lt_cap
 // > 0; its own instances, so the full-table instances keep their registers)
 template <int DIM, int HM, bool QUANT, bool AC, bool BND, typename R, int SINK, bool POS, bool LT>
-__global__ __launch_bounds__((HM == HM_LDS || (DIM == 3 && HM == HM_TMAJOR && !LT)) ? 1024 : 256) void k_trace(const LaunchParams p,
+__global__ __launch_bounds__((HM == HM_LDS || (DIM == 3 && HM == HM_TMAJOR && !LT) || (DIM == 2 && HM == HM_TMAJOR3 && !LT)) ? 1024 : 256) void k_trace(const LaunchParams p,
                                                                      const ProbDev* __restrict__ probs) {
   constexpr bool LDSH = HM == HM_LDS;
   constexpr bool TMAJ = HM == HM_TMAJOR || HM == HM_TMAJOR3;
